@@ -1,0 +1,202 @@
+/*
+ * nefi_wave.h — C ABI of the MI355X (gfx950) acoustic leapfrog library `libnefi_wave.so`.
+ *
+ * The reference path is pure Python: `WaveOperator.forward` / `simulate`
+ * (nefi/physics/wave.py:699-750), `WaveInitialConditionOperator.simulate` (wave.py:840-872),
+ * the step function `_LeapfrogBase._step_fn` (wave.py:520-554), the receiver gather `_sample`
+ * (wave.py:321-324), the time loop `run_timestepping` with block checkpointing
+ * (nefi/physics/timestep.py:78-169) and autograd's reverse sweep through that loop.
+ * Each entry point below replaces one of those pieces; the comment above it names which.
+ *
+ * Conventions
+ *  - Every buffer is device memory owned by the caller (the Python host allocates it with
+ *    PyTorch). The library owns only the plan object.
+ *  - Grids are row-major with axis 2 fastest. A d-dimensional padded grid (d = 1..3) is stored
+ *    as dims = (1,..,1, n_0, .., n_{d-1}); unused leading axes have extent 1 and no stencil.
+ *  - A wavefield buffer holds `n_shots` consecutive grids. PML auxiliary fields are face
+ *    grids (extent + 1 along their own axis), also `n_shots` consecutive.
+ *  - `dtype` selects float (0) or double (1) for every real-valued buffer of a plan.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t passed as void*; NULL = default
+ *    stream) and does not synchronise the host, except `nw_plan_create`.
+ *  - Every call returns 0 on success or a nonzero code; `nw_last_error()` returns the message
+ *    of the last failure on the calling thread. Nothing aborts the process.
+ */
+#ifndef NEFI_WAVE_H
+#define NEFI_WAVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NW_ABI_VERSION 1
+
+enum nw_status {
+  NW_OK = 0,
+  NW_ERR_ARG = 1,      /* invalid argument or descriptor */
+  NW_ERR_HIP = 2,      /* a HIP runtime call failed */
+  NW_ERR_UNSUPPORTED = 3
+};
+
+enum nw_absorbing { NW_NONE = 0, NW_SPONGE = 1, NW_CERJAN = 2, NW_PML = 3 };
+enum nw_dtype { NW_F32 = 0, NW_F64 = 1 };
+
+/* Plan descriptor: the geometry and constants that `_LeapfrogBase.__init__`/`_static`
+ * (wave.py:351-504) and `WaveOperator._extra_static` (wave.py:670-674) derive on the host. */
+typedef struct nw_desc {
+  int32_t ndim;        /* physical dimensions, 1..3 */
+  int32_t dtype;       /* nw_dtype */
+  int32_t order;       /* Laplacian accuracy order: 2 or 4 */
+  int32_t absorbing;   /* nw_absorbing */
+  int64_t dims[3];     /* padded grid extents (leading unused axes = 1) */
+  int64_t n_abs[3];    /* absorbing cells per side along each axis (0 for unused axes) */
+  int64_t field_dims[3]; /* physical (unpadded) grid = dims - 2*n_abs */
+  double h[3];         /* grid spacing per axis (1.0 for unused axes) */
+  double dt;           /* simulation time step */
+  int32_t n_steps;     /* leapfrog steps of one simulation */
+  int32_t n_t;         /* trace samples; sample j is taken after j*substeps steps */
+  int32_t substeps;
+  int32_t n_shots;     /* simulations advanced together (one source each, or none) */
+
+  /* Absorbing-layer profiles (device, `dtype`), one 1-D array per used axis, NULL otherwise.
+   * sponge, pml: damp[a][i] = dt/2 * rate_a(i) at cell centres (rate = σ_a or ζ_a);
+   * cerjan:      damp[a][i] = exp(-(gamma * xi_a(i))^2).
+   * pml only:    pml_alpha[a][f] = (1 - dt ζ_f/2)/(1 + dt ζ_f/2) at the dims[a]+1 faces,
+   *              pml_kappa[a][f] = dt / ((1 + dt ζ_f/2) h_a), pml_zf[a][f] = ζ_f,
+   *              pml_zc[a][i] = ζ at cell centres. */
+  const void* damp[3];
+  const void* pml_alpha[3];
+  const void* pml_kappa[3];
+  const void* pml_zf[3];
+  const void* pml_zc[3];
+
+  /* Sources (wave.py:312-318, 670-674, 720-721): per shot `src_taps` flat cell indices into
+   * the padded grid and their values delta/cell_volume*dt^2; the injected term at step n is
+   * src_val * wavelet[n]. src_taps = 0 means no source (initial-value problems). */
+  int32_t src_taps;
+  const int32_t* src_idx;  /* (n_shots, src_taps) */
+  const void* src_val;     /* (n_shots, src_taps) */
+  const void* wavelet;     /* (n_steps) */
+
+  /* Receivers (wave.py:272-300, 321-324): multilinear stencils of rec_taps = 2^ndim taps.
+   * Shot s samples the rows rec_first[s] .. rec_first[s]+rec_count[s]-1 of the tables and
+   * writes trace rows trace_first[s] ... of the (n_trace_rows, n_t) output. */
+  int32_t rec_taps;
+  int32_t rec_max;         /* max over shots of rec_count */
+  int32_t n_trace_rows;
+  const int32_t* rec_first;   /* (n_shots) */
+  const int32_t* rec_count;   /* (n_shots) */
+  const int32_t* trace_first; /* (n_shots) */
+  const int32_t* rec_idx;     /* (rows, rec_taps) flat cell index */
+  const void* rec_w;          /* (rows, rec_taps) */
+
+  /* Transpose of the receiver stencils, grouped by distinct cell so the reverse sweep adds
+   * residuals without atomics (deterministic): shot s touches cells
+   * inj_cell[inj_first[s] .. inj_first[s]+inj_count[s]-1]; entry range
+   * inj_off[u] .. inj_off[u+1]-1 of cell u lists (receiver r of the shot, weight). */
+  int32_t inj_max;            /* max over shots of inj_count */
+  const int32_t* inj_first;   /* (n_shots) */
+  const int32_t* inj_count;   /* (n_shots) */
+  const int32_t* inj_cell;    /* (U) */
+  const int32_t* inj_off;     /* (U+1) */
+  const int32_t* inj_rec;     /* (E) */
+  const void* inj_w;          /* (E) */
+} nw_desc;
+
+typedef struct nw_plan nw_plan;
+
+/* ABI version (NW_ABI_VERSION), sizeof(nw_desc) as compiled into the library (lets a binding
+ * check its struct layout), and the last error message of the calling thread. */
+int32_t nw_abi_version(void);
+int64_t nw_desc_bytes(void);
+const char* nw_last_error(void);
+
+/* Validate `desc` and keep a copy of it. Replaces `_LeapfrogBase._static` (wave.py:459-504):
+ * the constants stay 1-D profiles and tap lists instead of full padded grids. */
+int32_t nw_plan_create(const nw_desc* desc, nw_plan** out);
+int32_t nw_plan_destroy(nw_plan* plan);
+
+/* m = pad_replicate(c)^2 * dt^2 on the padded grid (wave.py:718, WaveGrid.pad wave.py:222-240).
+ * `c` has field_dims. If `ma` is not NULL it also receives m * a_eff, the model scaled by the
+ * damping factor of the step, which the reverse sweep uses. */
+int32_t nw_model(nw_plan* plan, const void* c, void* m, void* ma, void* stream);
+
+/* Advance steps n_begin .. n_end-1 of the leapfrog (wave.py:520-554 inside the loop of
+ * timestep.py:145-148), fusing the Laplacian, damping, source injection and the receiver
+ * gather of wave.py:723-729.
+ *
+ * state[0], state[1]: p^{n-1}, p^n of every shot on entry; on return the two pointers are
+ *   swapped as needed so that they again hold p^{n_end-1}, p^{n_end}.
+ * psi: PML only — psi[0..ndim-1] hold the auxiliary face fields psi^{n-1/2} and
+ *   psi[ndim..2ndim-1] are spare buffers of the same sizes; the pointer pairs are swapped on
+ *   return like `state`. NULL otherwise.
+ * traces: (n_trace_rows, n_t) output; sample j = n/substeps is written for every recorded
+ *   n in [n_begin, n_end] (p^{n_end} included). NULL skips sampling.
+ * ckpt / ckpt_every: if ckpt is not NULL, for every n in [n_begin, n_end) with
+ *   n % ckpt_every == 0 the state (p^{n-1}, p^n[, psi]) is written to checkpoint slot
+ *   n / ckpt_every (slot size = nw_ckpt_slot_bytes).
+ * keep / keep_begin: if keep is not NULL, p^{n} for n in [keep_begin, n_end] are also written
+ *   to keep slot n - keep_begin (slot = one wavefield buffer; PML: followed by the psi fields),
+ *   which the reverse sweep reads instead of recomputing them.
+ * first_step: the step index at which the simulation started (Cerjan tapers p^{n-1} only for
+ *   n > first_step). */
+int32_t nw_forward(nw_plan* plan, const void* m, void** state, void** psi, int32_t n_begin,
+                   int32_t n_end, void* traces, void* ckpt, int32_t ckpt_every, void* keep,
+                   int32_t keep_begin, int32_t first_step, void* stream);
+
+/* Bytes of one checkpoint slot and of one keep slot. */
+int64_t nw_ckpt_slot_bytes(const nw_plan* plan);
+int64_t nw_keep_slot_bytes(const nw_plan* plan);
+
+/* Reverse sweep over steps n_end-1 down to n_begin: the discrete adjoint of the leapfrog that
+ * autograd derives from wave.py:520-554 and replays through timestep.py:151-169.
+ *
+ * lam[0], lam[1]: adjoints of p^{n_end}, p^{n_end+1} on entry (zero at the end of time);
+ *   on return they hold the adjoints of p^{n_begin}, p^{n_begin+1}, pointers swapped as needed.
+ * psi_bar: PML only, 2*ndim buffers like `psi` of nw_forward (adjoints of psi^{n+1/2}).
+ * ma: m * a_eff from nw_model. m: the model itself.
+ * keep / keep_begin: p^n (and psi) for n in [keep_begin, ...] as written by nw_forward.
+ * residual: (n_trace_rows, n_t) = dJ/dtraces; injected at every recorded n in
+ *   [n_begin, n_end) (the caller injects n_end's sample before the first call via the same
+ *   rule: pass inject_end = 1).
+ * grad_m: if not NULL, per-shot float64 accumulators (n_shots, padded cells) receive
+ *   sum_n a_eff*lam^{n+1} * lap(p^n).
+ * first_step: as in nw_forward. */
+int32_t nw_adjoint(nw_plan* plan, const void* m, const void* ma, void** lam, void** psi_bar,
+                   int32_t n_begin, int32_t n_end, const void* keep, int32_t keep_begin,
+                   const void* residual, int32_t inject_end, double* grad_m,
+                   int32_t first_step, void* stream);
+
+/* dJ/dc = 2 dt^2 c * fold(sum_shots grad_m): the adjoint of m = pad_replicate(c)^2 dt^2, with
+ * the absorbing layer folded back onto the edge cells (adjoint of wave.py:222-240). `grad_c`
+ * has field_dims and the plan's dtype. */
+int32_t nw_model_grad(nw_plan* plan, const void* c, const double* grad_m, void* grad_c,
+                      void* stream);
+
+/* Initial state of the initial-value operator (wave.py:849-850): p^0 = pad_zeros(p0),
+ * p^{-1} = p^0 + m/2 * lap(p^0). `p0` is (n_shots, field_dims). */
+int32_t nw_ic_state(nw_plan* plan, const void* m, const void* p0, void* p_prev, void* p,
+                    void* stream);
+
+/* Adjoint of nw_ic_state: grad_p0 = crop(lam0 + pbar + lap(m/2 * pbar)) where lam0, pbar are
+ * the adjoints of p^0 and p^{-1} left by nw_adjoint. */
+int32_t nw_ic_grad(nw_plan* plan, const void* m, const void* lam0, const void* pbar,
+                   void* grad_p0, void* stream);
+
+/* Adjoint of p^{-1} after the last reverse step: pbar = -b * a_eff * lam^1. */
+int32_t nw_prev_adjoint(nw_plan* plan, const void* lam1, void* pbar, void* stream);
+
+/* out = lap(p) over `count` grids of the plan's padded shape, zero outside (wave.py:144-168). */
+int32_t nw_laplacian(nw_plan* plan, const void* p, void* out, int64_t count, void* stream);
+
+/* Set p[s, cells[k]] = values[k] for k < n_cells in every shot: the receiver clamp of
+ * time_reversal's Dirichlet mode (wave.py:958-962). */
+int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* values,
+                 int32_t n_cells, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEFI_WAVE_H */

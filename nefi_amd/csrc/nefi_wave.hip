@@ -1,0 +1,1020 @@
+// libnefi_wave: acoustic leapfrog propagator for MI355X (gfx950).
+//
+// Generic kernel family: any of 1-3 dimensions, Laplacian order 2 or 4, the four absorbing
+// layers of nefi (none, sponge, Cerjan taper, second-order PML), float or double. One thread
+// owns one padded cell of one shot and reads its stencil neighbours straight from global
+// memory (they are L1/L2 hits). The fast 2-D path lives in nefi_wave_2d.hip and is selected by
+// the driver for the shapes it supports.
+//
+// The step follows nefi/physics/wave.py:529-552 term by term:
+//   lap  = L p^n (+ PML: sum_a (G_a(psi'_a + psi_a)) / (2 h_a),  psi'_a = alpha psi_a + beta D_a p^n)
+//   rhs  = (2 p^n - b p^{n-1}) + m * lap  (- zz p^n in 2-D PML)  + src * w_n
+//   p^{n+1} = a * rhs                      (Cerjan: taper * rhs, and the stored p^n is tapered
+//                                           when it is read back as p^{n-1})
+// and the reverse sweep is its exact transpose (see k_adjoint).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "nefi_wave.h"
+#include "nefi_wave_internal.h"
+
+namespace nw {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// ------------------------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int64_t stride_of(const Geo<T>& g, int a) {
+  return a == 0 ? g.D[1] * g.D[2] : (a == 1 ? g.D[2] : 1);
+}
+
+// flat index of face f along axis a at the other coordinates of `co`
+template <typename T>
+__device__ __forceinline__ int64_t face_at(const Geo<T>& g, int a, const int64_t* co, int64_t f) {
+  const int64_t d1 = g.D[1] + (a == 1), d2 = g.D[2] + (a == 2);
+  const int64_t c0 = a == 0 ? f : co[0], c1 = a == 1 ? f : co[1], c2 = a == 2 ? f : co[2];
+  return (c0 * d1 + c1) * d2 + c2;
+}
+
+// damping factors of one cell: a_eff (applied to rhs), b (applied to p^{n-1}), taper
+template <typename T, int ABS>
+__device__ __forceinline__ void damping(const Geo<T>& g, const int64_t* co, T& a, T& b, T& tap) {
+  a = T(1);
+  b = T(1);
+  tap = T(1);
+  if (ABS == NW_SPONGE || ABS == NW_PML) {
+    T half = T(0);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      if (g.damp[ax]) half += g.damp[ax][co[ax]];
+    a = T(1) / (T(1) + half);
+    b = T(1) - half;
+  } else if (ABS == NW_CERJAN) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      if (g.damp[ax]) {
+        const T v = g.damp[ax][co[ax]];
+        tap = v < tap ? v : tap;
+      }
+    a = tap;
+  }
+}
+
+// stencil of a cell-valued function f(j) (j = flat index in the shot grid), zero outside
+template <typename T, int R, typename F>
+__device__ __forceinline__ T lap_of(const Geo<T>& g, const int64_t* co, int64_t idx, F f) {
+  T s = g.cc * f(idx);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!g.used[a]) continue;
+    const int64_t st = stride_of(g, a);
+    const int64_t i = co[a], n = g.D[a];
+    const T m1 = i >= 1 ? f(idx - st) : T(0);
+    const T p1 = i + 1 < n ? f(idx + st) : T(0);
+    T acc = g.c1[a] * (m1 + p1);
+    if (R == 2) {
+      const T m2 = i >= 2 ? f(idx - 2 * st) : T(0);
+      const T p2 = i + 2 < n ? f(idx + 2 * st) : T(0);
+      acc += g.c2[a] * (m2 + p2);
+    }
+    s += acc;
+  }
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ bool cell_of(const Geo<T>& g, int64_t& s, int64_t* co, int64_t& idx) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t y = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= g.D[2] || y >= g.D[1]) return false;
+  const int64_t zs = blockIdx.z;
+  co[0] = zs % g.D[0];
+  s = zs / g.D[0];
+  co[1] = y;
+  co[2] = x;
+  idx = (co[0] * g.D[1] + y) * g.D[2] + x;
+  return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// forward step
+// ------------------------------------------------------------------------------------------
+template <typename T, int R, int ABS>
+__global__ void __launch_bounds__(256) k_step(Geo<T> g, StepIO<T> io) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const int64_t base = s * g.cells;
+  const T* __restrict__ P = io.p + base;
+  const T pc = P[idx];
+  T lap = lap_of<T, R>(g, co, idx, [&](int64_t j) { return P[j]; });
+
+  if (ABS == NW_PML) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!g.used[a]) continue;
+      const int64_t st = stride_of(g, a), i = co[a], n = g.D[a];
+      const T pm = i >= 1 ? P[idx - st] : T(0);
+      const T pp = i + 1 < n ? P[idx + st] : T(0);
+      T other = T(0);
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if (b != a && g.used[b]) other += g.zc[b][co[b]];
+      const int64_t f0 = face_at(g, a, co, i), f1 = face_at(g, a, co, i + 1);
+      const int64_t fb = s * g.fcells[a];
+      const T o0 = io.psi_old[a][fb + f0], o1 = io.psi_old[a][fb + f1];
+      const T b0 = (other - g.zf[a][i]) * g.kappa[a][i];
+      const T b1 = (other - g.zf[a][i + 1]) * g.kappa[a][i + 1];
+      const T n0 = g.alpha[a][i] * o0 + b0 * (pc - pm);
+      const T n1 = g.alpha[a][i + 1] * o1 + b1 * (pp - pc);
+      lap += ((n1 + o1) - (n0 + o0)) * g.inv2h[a];
+      io.psi_new[a][fb + f0] = n0;
+      if (i == n - 1) io.psi_new[a][fb + f1] = n1;
+      if (io.ck_psi[a]) {
+        io.ck_psi[a][fb + f0] = o0;
+        if (i == n - 1) io.ck_psi[a][fb + f1] = o1;
+      }
+    }
+  }
+
+  T a_eff, b, tap;
+  damping<T, ABS>(g, co, a_eff, b, tap);
+  const T prev_raw = io.p_prev[base + idx];
+  const T prev = (ABS == NW_CERJAN && io.taper_prev) ? tap * prev_raw : prev_raw;
+  T rhs = (T(2) * pc - b * prev) + io.m[idx] * lap;
+  if (ABS == NW_PML && g.nd == 2) rhs -= g.dt2 * g.zc[1][co[1]] * g.zc[2][co[2]] * pc;
+  if (io.src_taps > 0) {
+    const T w = io.wavelet[io.n];
+    for (int k = 0; k < io.src_taps; ++k)
+      if (io.src_idx[s * io.src_taps + k] == idx) rhs += io.src_val[s * io.src_taps + k] * w;
+  }
+  io.p_next[base + idx] = (ABS == NW_NONE) ? rhs : a_eff * rhs;
+  if (io.ck_prev) {
+    io.ck_prev[base + idx] = prev_raw;
+    io.ck_cur[base + idx] = pc;
+  }
+}
+
+// receiver gather of p^n (wave.py:321-324): groups of `taps` lanes hold one receiver's taps
+// and are reduced with wavefront shuffles; lane 0 of the group writes the sample.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sample(Geo<T> g, SampleIO<T> io) {
+  const int s = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int taps = io.taps;
+  const int r = t / taps, k = t % taps;
+  const int count = io.rec_count[s];
+  T v = T(0);
+  if (r < count) {
+    const int64_t row = (int64_t)io.rec_first[s] + r;
+    v = io.rec_w[row * taps + k] * io.p[(int64_t)s * g.cells + io.rec_idx[row * taps + k]];
+  }
+  for (int off = 1; off < taps; off <<= 1) v += __shfl_xor(v, off, 64);
+  if (r < count && k == 0)
+    io.traces[((int64_t)io.trace_first[s] + r) * io.n_t + io.j] = v;
+}
+
+// lam^n += R^T g_n for one recorded step (deterministic: one thread per distinct cell)
+template <typename T>
+__global__ void __launch_bounds__(256) k_inject(Geo<T> g, InjectIO<T> io) {
+  const int s = blockIdx.y;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= io.inj_count[s]) return;
+  const int uu = io.inj_first[s] + u;
+  const int64_t row0 = io.trace_first[s];
+  T acc = T(0);
+  for (int e = io.inj_off[uu]; e < io.inj_off[uu + 1]; ++e)
+    acc += io.inj_w[e] * io.residual[(row0 + io.inj_rec[e]) * io.n_t + io.j];
+  io.lam[(int64_t)s * g.cells + io.inj_cell[uu]] += acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// reverse step: lam^n from lam^{n+1}, lam^{n+2} (transpose of k_step), plus the imaging sum
+// ------------------------------------------------------------------------------------------
+template <typename T, int R, int ABS>
+__global__ void __launch_bounds__(256) k_adjoint(Geo<T> g, AdjIO<T> io) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const int64_t base = s * g.cells;
+  const T* __restrict__ L1 = io.lam1 + base;
+  const T* __restrict__ MA = io.ma;
+  T a_eff, b, tap;
+  damping<T, ABS>(g, co, a_eff, b, tap);
+  const T rbar = a_eff * L1[idx];
+  // adjoint of the p^{n-1} slot of step n+1: -b * t_p(n+1) * a_eff * lam^{n+2}
+  const T tp1 = (ABS == NW_CERJAN && io.taper_next) ? tap : T(1);
+  const T pbar_next = -(b * tp1) * (a_eff * io.lam2[base + idx]);
+  auto q = [&](int64_t j) { return MA[j] * L1[j]; };
+  T zz = T(0);
+  if (ABS == NW_PML && g.nd == 2) zz = g.dt2 * g.zc[1][co[1]] * g.zc[2][co[2]];
+  T lam = pbar_next + (T(2) - zz) * rbar + lap_of<T, R>(g, co, idx, q);
+
+  T lap_p = T(0);
+  const T* __restrict__ Pn = io.p_n ? io.p_n + base : nullptr;
+  if (Pn) lap_p = lap_of<T, R>(g, co, idx, [&](int64_t j) { return Pn[j]; });
+
+  if (ABS == NW_PML) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!g.used[a]) continue;
+      const int64_t st = stride_of(g, a), i = co[a], n = g.D[a];
+      const T qc = q(idx);
+      const T qm = i >= 1 ? q(idx - st) : T(0);
+      const T qp = i + 1 < n ? q(idx + st) : T(0);
+      const T dq0 = qc - qm, dq1 = qp - qc;
+      T other = T(0);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+        if (bb != a && g.used[bb]) other += g.zc[bb][co[bb]];
+      const int64_t f0 = face_at(g, a, co, i), f1 = face_at(g, a, co, i + 1);
+      const int64_t fb = s * g.fcells[a];
+      const T t0 = io.psib_in[a][fb + f0] - g.inv2h[a] * dq0;
+      const T t1 = io.psib_in[a][fb + f1] - g.inv2h[a] * dq1;
+      const T b0 = (other - g.zf[a][i]) * g.kappa[a][i];
+      const T b1 = (other - g.zf[a][i + 1]) * g.kappa[a][i + 1];
+      lam -= b1 * t1 - b0 * t0;
+      io.psib_out[a][fb + f0] = g.alpha[a][i] * t0 - g.inv2h[a] * dq0;
+      if (i == n - 1) io.psib_out[a][fb + f1] = g.alpha[a][i + 1] * t1 - g.inv2h[a] * dq1;
+      if (Pn) {
+        const T* ph = io.psi_half[a] + fb;  // psi^{n+1/2}
+        const T* pl = io.psi_low[a] + fb;   // psi^{n-1/2}
+        lap_p += ((ph[f1] + pl[f1]) - (ph[f0] + pl[f0])) * g.inv2h[a];
+      }
+    }
+  }
+  io.lam0[base + idx] = lam;
+  if (Pn && io.grad) io.grad[base + idx] += (double)rbar * (double)lap_p;
+}
+
+// ------------------------------------------------------------------------------------------
+// model, gradient fold-back, initial-value state, utilities
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int64_t clamp_field(int64_t i, int64_t a, int64_t n) {
+  const int64_t j = i - a;
+  return j < 0 ? 0 : (j >= n ? n - 1 : j);
+}
+
+template <typename T, int ABS>
+__global__ void __launch_bounds__(256) k_model(Geo<T> g, const T* __restrict__ c, T* m, T* ma) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const int64_t i0 = clamp_field<T>(co[0], g.nabs[0], g.F[0]);
+  const int64_t i1 = clamp_field<T>(co[1], g.nabs[1], g.F[1]);
+  const int64_t i2 = clamp_field<T>(co[2], g.nabs[2], g.F[2]);
+  const T cv = c[(i0 * g.F[1] + i1) * g.F[2] + i2];
+  const T mv = (cv * cv) * g.dt2;
+  m[idx] = mv;
+  if (ma) {
+    T a_eff, b, tap;
+    damping<T, ABS>(g, co, a_eff, b, tap);
+    ma[idx] = mv * a_eff;
+  }
+}
+
+// range of padded cells along one axis that replicate-padding maps onto field cell i
+__device__ __forceinline__ void fold_range(int64_t i, int64_t a, int64_t n, int64_t& lo,
+                                           int64_t& hi) {
+  lo = i == 0 ? 0 : a + i;
+  hi = i == n - 1 ? 2 * a + n - 1 : a + i;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_model_grad(Geo<T> g, const T* __restrict__ c,
+                                                    const double* __restrict__ gm, T* gc,
+                                                    int64_t shots) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nf = g.F[0] * g.F[1] * g.F[2];
+  if (t >= nf) return;
+  const int64_t i2 = t % g.F[2], i1 = (t / g.F[2]) % g.F[1], i0 = t / (g.F[1] * g.F[2]);
+  int64_t lo0, hi0, lo1, hi1, lo2, hi2;
+  fold_range(i0, g.nabs[0], g.F[0], lo0, hi0);
+  fold_range(i1, g.nabs[1], g.F[1], lo1, hi1);
+  fold_range(i2, g.nabs[2], g.F[2], lo2, hi2);
+  double acc = 0.0;
+  for (int64_t s = 0; s < shots; ++s) {
+    const double* G = gm + s * g.cells;
+    for (int64_t z = lo0; z <= hi0; ++z)
+      for (int64_t y = lo1; y <= hi1; ++y)
+        for (int64_t x = lo2; x <= hi2; ++x) acc += G[(z * g.D[1] + y) * g.D[2] + x];
+  }
+  gc[t] = (T)(2.0 * (double)g.dt2_d * (double)c[t] * acc);
+}
+
+template <typename T>
+__device__ __forceinline__ T p0_at(const Geo<T>& g, const T* p0s, int64_t j) {
+  const int64_t x = j % g.D[2], y = (j / g.D[2]) % g.D[1], z = j / (g.D[1] * g.D[2]);
+  const int64_t i0 = z - g.nabs[0], i1 = y - g.nabs[1], i2 = x - g.nabs[2];
+  if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= g.F[0] || i1 >= g.F[1] || i2 >= g.F[2]) return T(0);
+  return p0s[(i0 * g.F[1] + i1) * g.F[2] + i2];
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_ic_state(Geo<T> g, const T* __restrict__ m,
+                                                  const T* __restrict__ p0, T* p_prev, T* p) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const int64_t nf = g.F[0] * g.F[1] * g.F[2];
+  const T* p0s = p0 + s * nf;
+  const T pc = p0_at(g, p0s, idx);
+  const T lap = lap_of<T, R>(g, co, idx, [&](int64_t j) { return p0_at(g, p0s, j); });
+  p[s * g.cells + idx] = pc;
+  p_prev[s * g.cells + idx] = pc + (T(0.5) * m[idx]) * lap;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_ic_grad(Geo<T> g, const T* __restrict__ m,
+                                                 const T* __restrict__ lam0,
+                                                 const T* __restrict__ pbar, T* gp0) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const int64_t i0 = co[0] - g.nabs[0], i1 = co[1] - g.nabs[1], i2 = co[2] - g.nabs[2];
+  if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= g.F[0] || i1 >= g.F[1] || i2 >= g.F[2]) return;
+  const T* PB = pbar + s * g.cells;
+  const T lap = lap_of<T, R>(g, co, idx, [&](int64_t j) { return (T(0.5) * m[j]) * PB[j]; });
+  const int64_t nf = g.F[0] * g.F[1] * g.F[2];
+  gp0[s * nf + (i0 * g.F[1] + i1) * g.F[2] + i2] = lam0[s * g.cells + idx] + PB[idx] + lap;
+}
+
+template <typename T, int ABS>
+__global__ void __launch_bounds__(256) k_prev_adjoint(Geo<T> g, const T* __restrict__ lam1,
+                                                      T* pbar) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  T a_eff, b, tap;
+  damping<T, ABS>(g, co, a_eff, b, tap);
+  pbar[s * g.cells + idx] = -b * (a_eff * lam1[s * g.cells + idx]);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_laplacian(Geo<T> g, const T* __restrict__ p, T* out) {
+  int64_t s, co[3], idx;
+  if (!cell_of(g, s, co, idx)) return;
+  const T* P = p + s * g.cells;
+  out[s * g.cells + idx] = lap_of<T, R>(g, co, idx, [&](int64_t j) { return P[j]; });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_clamp(int64_t cells, T* p, const int32_t* __restrict__ idx,
+                                               const T* __restrict__ val, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  p[(int64_t)blockIdx.y * cells + idx[k]] = val[k];
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+template <typename T>
+Geo<T> make_geo(const nw_desc& d) {
+  Geo<T> g{};
+  g.nd = d.ndim;
+  for (int a = 0; a < 3; ++a) {
+    g.D[a] = d.dims[a];
+    g.F[a] = d.field_dims[a];
+    g.nabs[a] = d.n_abs[a];
+    g.used[a] = a >= 3 - d.ndim;
+  }
+  g.cells = g.D[0] * g.D[1] * g.D[2];
+  for (int a = 0; a < 3; ++a) {
+    int64_t f = 1;
+    for (int b = 0; b < 3; ++b) f *= g.D[b] + (b == a);
+    g.fcells[a] = f;
+  }
+  // coefficients of nefi/physics/wave.py:121 per axis, summed into the centre in f64
+  const double c0 = d.order == 4 ? -2.5 : -2.0;
+  const double c1 = d.order == 4 ? 4.0 / 3.0 : 1.0;
+  const double c2 = d.order == 4 ? -1.0 / 12.0 : 0.0;
+  double cc = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    if (!g.used[a]) continue;
+    const double ih2 = 1.0 / (d.h[a] * d.h[a]);
+    cc += c0 * ih2;
+    g.c1[a] = (T)(c1 * ih2);
+    g.c2[a] = (T)(c2 * ih2);
+    g.inv2h[a] = (T)(0.5 / d.h[a]);
+  }
+  g.cc = (T)cc;
+  g.dt2 = (T)(d.dt * d.dt);
+  g.dt2_d = d.dt * d.dt;
+  for (int a = 0; a < 3; ++a) {
+    g.damp[a] = (const T*)d.damp[a];
+    g.alpha[a] = (const T*)d.pml_alpha[a];
+    g.kappa[a] = (const T*)d.pml_kappa[a];
+    g.zf[a] = (const T*)d.pml_zf[a];
+    g.zc[a] = (const T*)d.pml_zc[a];
+  }
+  return g;
+}
+
+}  // namespace nw
+
+using namespace nw;
+
+struct nw_plan {
+  nw_desc d;
+  Geo<float> gf;
+  Geo<double> gd;
+  int64_t shot_bytes;          // one wavefield (all shots)
+  int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
+};
+
+namespace nw {
+
+static size_t esize(const nw_plan* p) { return p->d.dtype == NW_F64 ? 8 : 4; }
+
+static bool recorded(const nw_desc& d, int n) {
+  return n % d.substeps == 0 && n / d.substeps < d.n_t;
+}
+
+static dim3 block_for(const int64_t* D) { return D[1] > 1 ? dim3(64, 4, 1) : dim3(256, 1, 1); }
+
+static dim3 grid_for(const int64_t* D, int64_t shots, dim3 blk) {
+  return dim3((unsigned)((D[2] + blk.x - 1) / blk.x), (unsigned)((D[1] + blk.y - 1) / blk.y),
+              (unsigned)(D[0] * shots));
+}
+
+static int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(NW_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return NW_OK;
+}
+
+#define NW_TRY(x)            \
+  do {                       \
+    const int rc_ = (x);     \
+    if (rc_ != NW_OK) return rc_; \
+  } while (0)
+
+#define NW_HIP(call)                                                                   \
+  do {                                                                                 \
+    const hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) return fail(NW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static const Geo<T>& geo(const nw_plan* p);
+template <>
+const Geo<float>& geo<float>(const nw_plan* p) { return p->gf; }
+template <>
+const Geo<double>& geo<double>(const nw_plan* p) { return p->gd; }
+
+template <typename T>
+static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t st) {
+  const nw_desc& d = P->d;
+  if (d.rec_max <= 0) return NW_OK;
+  SampleIO<T> io{};
+  io.p = p;
+  io.traces = traces;
+  io.j = j;
+  io.n_t = d.n_t;
+  io.taps = d.rec_taps;
+  io.rec_first = d.rec_first;
+  io.rec_count = d.rec_count;
+  io.trace_first = d.trace_first;
+  io.rec_idx = d.rec_idx;
+  io.rec_w = (const T*)d.rec_w;
+  const int threads = d.rec_max * d.rec_taps;
+  dim3 grid((threads + 255) / 256, d.n_shots);
+  hipLaunchKernelGGL(k_sample<T>, grid, dim3(256), 0, st, geo<T>(P), io);
+  return check_launch("k_sample");
+}
+
+template <typename T>
+static int launch_inject(nw_plan* P, T* lam, const T* residual, int j, hipStream_t st) {
+  const nw_desc& d = P->d;
+  if (d.inj_max <= 0) return NW_OK;
+  InjectIO<T> io{};
+  io.lam = lam;
+  io.residual = residual;
+  io.j = j;
+  io.n_t = d.n_t;
+  io.inj_first = d.inj_first;
+  io.inj_count = d.inj_count;
+  io.inj_cell = d.inj_cell;
+  io.inj_off = d.inj_off;
+  io.inj_rec = d.inj_rec;
+  io.inj_w = (const T*)d.inj_w;
+  io.trace_first = d.trace_first;
+  dim3 grid((d.inj_max + 255) / 256, d.n_shots);
+  hipLaunchKernelGGL(k_inject<T>, grid, dim3(256), 0, st, geo<T>(P), io);
+  return check_launch("k_inject");
+}
+
+template <typename T, int R, int ABS>
+static void launch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
+  const Geo<T>& g = geo<T>(P);
+  const dim3 blk = block_for(g.D);
+  hipLaunchKernelGGL((k_step<T, R, ABS>), grid_for(g.D, P->d.n_shots, blk), blk, 0, st, g, io);
+}
+
+template <typename T, int R, int ABS>
+static void launch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
+  const Geo<T>& g = geo<T>(P);
+  const dim3 blk = block_for(g.D);
+  hipLaunchKernelGGL((k_adjoint<T, R, ABS>), grid_for(g.D, P->d.n_shots, blk), blk, 0, st, g,
+                     io);
+}
+
+template <typename T>
+static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
+  const int r = P->d.order == 4 ? 2 : 1;
+  switch (P->d.absorbing * 2 + (r - 1)) {
+    case NW_NONE * 2 + 0: launch_step<T, 1, NW_NONE>(P, io, st); break;
+    case NW_NONE * 2 + 1: launch_step<T, 2, NW_NONE>(P, io, st); break;
+    case NW_SPONGE * 2 + 0: launch_step<T, 1, NW_SPONGE>(P, io, st); break;
+    case NW_SPONGE * 2 + 1: launch_step<T, 2, NW_SPONGE>(P, io, st); break;
+    case NW_CERJAN * 2 + 0: launch_step<T, 1, NW_CERJAN>(P, io, st); break;
+    case NW_CERJAN * 2 + 1: launch_step<T, 2, NW_CERJAN>(P, io, st); break;
+    case NW_PML * 2 + 0: launch_step<T, 1, NW_PML>(P, io, st); break;
+    default: launch_step<T, 2, NW_PML>(P, io, st); break;
+  }
+}
+
+template <typename T>
+static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
+  const int r = P->d.order == 4 ? 2 : 1;
+  switch (P->d.absorbing * 2 + (r - 1)) {
+    case NW_NONE * 2 + 0: launch_adj<T, 1, NW_NONE>(P, io, st); break;
+    case NW_NONE * 2 + 1: launch_adj<T, 2, NW_NONE>(P, io, st); break;
+    case NW_SPONGE * 2 + 0: launch_adj<T, 1, NW_SPONGE>(P, io, st); break;
+    case NW_SPONGE * 2 + 1: launch_adj<T, 2, NW_SPONGE>(P, io, st); break;
+    case NW_CERJAN * 2 + 0: launch_adj<T, 1, NW_CERJAN>(P, io, st); break;
+    case NW_CERJAN * 2 + 1: launch_adj<T, 2, NW_CERJAN>(P, io, st); break;
+    case NW_PML * 2 + 0: launch_adj<T, 1, NW_PML>(P, io, st); break;
+    default: launch_adj<T, 2, NW_PML>(P, io, st); break;
+  }
+}
+
+// pointers of the PML face fields inside a keep slot
+template <typename T>
+static void keep_psi(const nw_plan* P, T* slot, T** psi) {
+  char* q = (char*)slot + P->shot_bytes;
+  for (int a = 0; a < 3; ++a) {
+    psi[a] = nullptr;
+    if (P->face_bytes[a]) {
+      psi[a] = (T*)q;
+      q += P->face_bytes[a];
+    }
+  }
+}
+
+template <typename T>
+static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_begin, int n_end,
+                        T* traces, T* ckpt, int ckpt_every, T* keep, int keep_begin,
+                        int first_step, hipStream_t st) {
+  const nw_desc& d = P->d;
+  const bool pml = d.absorbing == NW_PML;
+  const int64_t keep_slot = nw_keep_slot_bytes(P);
+  const int64_t ck_slot = nw_ckpt_slot_bytes(P);
+  T* s0 = (T*)state[0];
+  T* s1 = (T*)state[1];
+  // psi pairs per 3-D axis: current and spare
+  T* pa[3] = {nullptr, nullptr, nullptr};
+  T* pb[3] = {nullptr, nullptr, nullptr};
+  if (pml) {
+    int k = 0;
+    for (int a = 0; a < 3; ++a)
+      if (P->face_bytes[a]) {
+        pa[a] = (T*)psi[k];
+        pb[a] = (T*)psi[d.ndim + k];
+        ++k;
+      }
+  }
+  auto kslot = [&](int n) { return (T*)((char*)keep + (int64_t)(n - keep_begin) * keep_slot); };
+  const bool keeping = keep != nullptr;
+  if (keeping && keep_begin == n_begin) {
+    NW_HIP(hipMemcpyAsync(kslot(n_begin), s1, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+    if (pml) {
+      T* kp[3];
+      keep_psi(P, kslot(n_begin), kp);
+      for (int a = 0; a < 3; ++a)
+        if (pa[a]) NW_HIP(hipMemcpyAsync(kp[a], pa[a], P->face_bytes[a], hipMemcpyDeviceToDevice, st));
+    }
+  }
+  // current pointers (may point into keep slots)
+  T* prev = s0;
+  T* cur = s1;
+  T* psi_cur[3] = {pa[0], pa[1], pa[2]};
+  for (int n = n_begin; n < n_end; ++n) {
+    if (traces && recorded(d, n)) NW_TRY(launch_sample<T>(P, cur, traces, n / d.substeps, st));
+    StepIO<T> io{};
+    io.m = m;
+    io.p_prev = prev;
+    io.p = cur;
+    T* next;
+    T* psi_next[3] = {nullptr, nullptr, nullptr};
+    if (keeping) {
+      next = kslot(n + 1);
+      if (pml) keep_psi(P, next, psi_next);
+    } else {
+      next = (prev == s0 || prev == s1) ? prev : (cur == s0 ? s1 : s0);
+      for (int a = 0; a < 3; ++a)
+        if (pa[a]) psi_next[a] = (psi_cur[a] == pa[a]) ? pb[a] : pa[a];
+    }
+    io.p_next = next;
+    for (int a = 0; a < 3; ++a) {
+      io.psi_old[a] = psi_cur[a];
+      io.psi_new[a] = psi_next[a];
+    }
+    io.taper_prev = n > first_step;
+    if (d.src_taps > 0 && n < d.n_steps) {
+      io.src_taps = d.src_taps;
+      io.src_idx = d.src_idx;
+      io.src_val = (const T*)d.src_val;
+      io.wavelet = (const T*)d.wavelet;
+      io.n = n;
+    }
+    if (ckpt && ckpt_every > 0 && n % ckpt_every == 0) {
+      T* slot = (T*)((char*)ckpt + (int64_t)(n / ckpt_every) * ck_slot);
+      io.ck_prev = slot;
+      io.ck_cur = (T*)((char*)slot + P->shot_bytes);
+      char* q = (char*)slot + 2 * P->shot_bytes;
+      for (int a = 0; a < 3; ++a)
+        if (P->face_bytes[a]) {
+          io.ck_psi[a] = (T*)q;
+          q += P->face_bytes[a];
+        }
+    }
+    dispatch_step<T>(P, io, st);
+    NW_TRY(check_launch("k_step"));
+    prev = cur;
+    cur = next;
+    for (int a = 0; a < 3; ++a) psi_cur[a] = psi_next[a];
+  }
+  if (traces && recorded(d, n_end)) NW_TRY(launch_sample<T>(P, cur, traces, n_end / d.substeps, st));
+  // hand the final state back in the caller's buffers
+  if (n_end > n_begin) {
+    const bool prev_own = prev == s0 || prev == s1;
+    const bool cur_own = cur == s0 || cur == s1;
+    if (prev_own && cur_own) {
+      state[0] = prev;
+      state[1] = cur;
+    } else {
+      // at least one lives in a keep slot: copy into the caller's buffers
+      if (cur_own) {
+        T* other = cur == s0 ? s1 : s0;
+        NW_HIP(hipMemcpyAsync(other, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+        state[0] = other;
+        state[1] = cur;
+      } else if (prev_own) {
+        T* other = prev == s0 ? s1 : s0;
+        NW_HIP(hipMemcpyAsync(other, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+        state[0] = prev;
+        state[1] = other;
+      } else {
+        NW_HIP(hipMemcpyAsync(s0, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+        NW_HIP(hipMemcpyAsync(s1, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+        state[0] = s0;
+        state[1] = s1;
+      }
+    }
+    if (pml) {
+      int k = 0;
+      for (int a = 0; a < 3; ++a) {
+        if (!pa[a]) continue;
+        T* c = psi_cur[a];
+        if (c != pa[a] && c != pb[a]) {
+          NW_HIP(hipMemcpyAsync(pa[a], c, P->face_bytes[a], hipMemcpyDeviceToDevice, st));
+          c = pa[a];
+        }
+        psi[k] = c;
+        psi[d.ndim + k] = c == pa[a] ? pb[a] : pa[a];
+        ++k;
+      }
+    }
+  }
+  return NW_OK;
+}
+
+template <typename T>
+static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** psib, int n_begin,
+                        int n_end, const T* keep, int keep_begin, const T* residual,
+                        int inject_end, double* grad, int first_step, hipStream_t st) {
+  (void)m;
+  const nw_desc& d = P->d;
+  const bool pml = d.absorbing == NW_PML;
+  const int64_t keep_slot = nw_keep_slot_bytes(P);
+  T* L1 = (T*)lam[0];
+  T* L2 = (T*)lam[1];
+  T* qa[3] = {nullptr, nullptr, nullptr};
+  T* qb[3] = {nullptr, nullptr, nullptr};
+  if (pml) {
+    int k = 0;
+    for (int a = 0; a < 3; ++a)
+      if (P->face_bytes[a]) {
+        qa[a] = (T*)psib[k];
+        qb[a] = (T*)psib[d.ndim + k];
+        ++k;
+      }
+  }
+  auto kslot = [&](int n) {
+    return (const T*)((const char*)keep + (int64_t)(n - keep_begin) * keep_slot);
+  };
+  if (residual && inject_end && recorded(d, n_end))
+    NW_TRY(launch_inject<T>(P, L1, residual, n_end / d.substeps, st));
+  for (int n = n_end - 1; n >= n_begin; --n) {
+    AdjIO<T> io{};
+    io.ma = ma;
+    io.lam1 = L1;
+    io.lam2 = L2;
+    io.lam0 = L2;
+    io.taper_next = (n + 1) > first_step;
+    if (keep) {
+      io.p_n = kslot(n);
+      io.grad = grad;
+      if (pml) {
+        T* lo[3];
+        T* hi[3];
+        keep_psi(P, (T*)kslot(n), lo);
+        keep_psi(P, (T*)kslot(n + 1), hi);
+        for (int a = 0; a < 3; ++a) {
+          io.psi_low[a] = lo[a];
+          io.psi_half[a] = hi[a];
+        }
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      io.psib_in[a] = qa[a];
+      io.psib_out[a] = qb[a];
+    }
+    dispatch_adj<T>(P, io, st);
+    NW_TRY(check_launch("k_adjoint"));
+    T* t = L1;
+    L1 = L2;  // now lam^n
+    L2 = t;   // lam^{n+1}
+    for (int a = 0; a < 3; ++a) {
+      T* u = qa[a];
+      qa[a] = qb[a];
+      qb[a] = u;
+    }
+    if (residual && recorded(d, n)) NW_TRY(launch_inject<T>(P, L1, residual, n / d.substeps, st));
+  }
+  lam[0] = L1;
+  lam[1] = L2;
+  if (pml) {
+    int k = 0;
+    for (int a = 0; a < 3; ++a)
+      if (qa[a]) {
+        psib[k] = qa[a];
+        psib[d.ndim + k] = qb[a];
+        ++k;
+      }
+  }
+  return NW_OK;
+}
+
+}  // namespace nw
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int32_t nw_abi_version(void) { return NW_ABI_VERSION; }
+
+int64_t nw_desc_bytes(void) { return (int64_t)sizeof(nw_desc); }
+
+const char* nw_last_error(void) { return nw::g_err.c_str(); }
+
+int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
+  if (!desc || !out) return fail(NW_ERR_ARG, "nw_plan_create: null argument");
+  const nw_desc& d = *desc;
+  if (d.ndim < 1 || d.ndim > 3) return fail(NW_ERR_ARG, "ndim must be 1..3");
+  if (d.order != 2 && d.order != 4) return fail(NW_ERR_ARG, "order must be 2 or 4");
+  if (d.absorbing < NW_NONE || d.absorbing > NW_PML) return fail(NW_ERR_ARG, "bad absorbing");
+  if (d.dtype != NW_F32 && d.dtype != NW_F64) return fail(NW_ERR_ARG, "bad dtype");
+  if (d.absorbing == NW_PML && d.ndim > 2) return fail(NW_ERR_ARG, "PML supports 1-D/2-D");
+  if (d.n_shots < 1 || d.substeps < 1 || d.n_t < 1 || d.n_steps < 0)
+    return fail(NW_ERR_ARG, "bad time axis or shot count");
+  for (int a = 0; a < 3; ++a) {
+    if (d.dims[a] < 1) return fail(NW_ERR_ARG, "dims must be >= 1");
+    if (a < 3 - d.ndim && d.dims[a] != 1) return fail(NW_ERR_ARG, "unused leading dims must be 1");
+    if (d.field_dims[a] + 2 * d.n_abs[a] != d.dims[a])
+      return fail(NW_ERR_ARG, "dims must equal field_dims + 2*n_abs");
+  }
+  const int64_t cells = d.dims[0] * d.dims[1] * d.dims[2];
+  if (cells >= (int64_t)1 << 31) return fail(NW_ERR_ARG, "padded grid exceeds 2^31 cells");
+  if (d.dims[0] * d.n_shots > 65535) return fail(NW_ERR_ARG, "dims[0] * n_shots exceeds 65535");
+  if (d.rec_max > 0 && (d.rec_taps != (1 << d.ndim) || !d.rec_idx || !d.rec_w))
+    return fail(NW_ERR_ARG, "receiver tables inconsistent with ndim");
+  if (d.src_taps > 0 && (!d.src_idx || !d.src_val || !d.wavelet))
+    return fail(NW_ERR_ARG, "source tables missing");
+  for (int a = 0; a < 3; ++a) {
+    const bool used = a >= 3 - d.ndim;
+    const bool need_damp = used && d.absorbing != NW_NONE;
+    if (need_damp && !d.damp[a]) return fail(NW_ERR_ARG, "damping profile missing");
+    if (used && d.absorbing == NW_PML &&
+        (!d.pml_alpha[a] || !d.pml_kappa[a] || !d.pml_zf[a] || !d.pml_zc[a]))
+      return fail(NW_ERR_ARG, "PML profiles missing");
+  }
+  nw_plan* p = new nw_plan();
+  p->d = d;
+  p->gf = make_geo<float>(d);
+  p->gd = make_geo<double>(d);
+  const size_t es = d.dtype == NW_F64 ? 8 : 4;
+  p->shot_bytes = cells * (int64_t)es * d.n_shots;
+  for (int a = 0; a < 3; ++a)
+    p->face_bytes[a] = (d.absorbing == NW_PML && a >= 3 - d.ndim)
+                           ? p->gf.fcells[a] * (int64_t)es * d.n_shots
+                           : 0;
+  *out = p;
+  return NW_OK;
+}
+
+int32_t nw_plan_destroy(nw_plan* plan) {
+  delete plan;
+  return NW_OK;
+}
+
+int64_t nw_ckpt_slot_bytes(const nw_plan* P) {
+  return 2 * P->shot_bytes + P->face_bytes[0] + P->face_bytes[1] + P->face_bytes[2];
+}
+
+int64_t nw_keep_slot_bytes(const nw_plan* P) {
+  return P->shot_bytes + P->face_bytes[0] + P->face_bytes[1] + P->face_bytes[2];
+}
+
+int32_t nw_model(nw_plan* P, const void* c, void* m, void* ma, void* stream) {
+  if (!P || !c || !m) return fail(NW_ERR_ARG, "nw_model: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* D = P->d.dims;
+  const dim3 blk = block_for(D);
+  const dim3 grd = grid_for(D, 1, blk);
+#define NW_MODEL(T, A) \
+  hipLaunchKernelGGL((k_model<T, A>), grd, blk, 0, st, geo<T>(P), (const T*)c, (T*)m, (T*)ma)
+  if (P->d.dtype == NW_F32) {
+    switch (P->d.absorbing) {
+      case NW_NONE: NW_MODEL(float, NW_NONE); break;
+      case NW_SPONGE: NW_MODEL(float, NW_SPONGE); break;
+      case NW_CERJAN: NW_MODEL(float, NW_CERJAN); break;
+      default: NW_MODEL(float, NW_PML); break;
+    }
+  } else {
+    switch (P->d.absorbing) {
+      case NW_NONE: NW_MODEL(double, NW_NONE); break;
+      case NW_SPONGE: NW_MODEL(double, NW_SPONGE); break;
+      case NW_CERJAN: NW_MODEL(double, NW_CERJAN); break;
+      default: NW_MODEL(double, NW_PML); break;
+    }
+  }
+#undef NW_MODEL
+  return check_launch("k_model");
+}
+
+int32_t nw_forward(nw_plan* P, const void* m, void** state, void** psi, int32_t n_begin,
+                   int32_t n_end, void* traces, void* ckpt, int32_t ckpt_every, void* keep,
+                   int32_t keep_begin, int32_t first_step, void* stream) {
+  if (!P || !m || !state || !state[0] || !state[1]) return fail(NW_ERR_ARG, "nw_forward: null argument");
+  if (n_begin < 0 || n_end < n_begin || n_end > P->d.n_steps)
+    return fail(NW_ERR_ARG, "nw_forward: step range outside [0, n_steps]");
+  if (P->d.absorbing == NW_PML && !psi) return fail(NW_ERR_ARG, "nw_forward: PML needs psi buffers");
+  if (keep && keep_begin != n_begin) return fail(NW_ERR_ARG, "nw_forward: keep_begin must equal n_begin");
+  if (ckpt && ckpt_every < 1) return fail(NW_ERR_ARG, "nw_forward: ckpt_every must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  if (P->d.dtype == NW_F32)
+    return forward_impl<float>(P, (const float*)m, state, psi, n_begin, n_end, (float*)traces,
+                               (float*)ckpt, ckpt_every, (float*)keep, keep_begin, first_step, st);
+  return forward_impl<double>(P, (const double*)m, state, psi, n_begin, n_end, (double*)traces,
+                              (double*)ckpt, ckpt_every, (double*)keep, keep_begin, first_step, st);
+}
+
+int32_t nw_adjoint(nw_plan* P, const void* m, const void* ma, void** lam, void** psi_bar,
+                   int32_t n_begin, int32_t n_end, const void* keep, int32_t keep_begin,
+                   const void* residual, int32_t inject_end, double* grad_m, int32_t first_step,
+                   void* stream) {
+  if (!P || !ma || !lam || !lam[0] || !lam[1]) return fail(NW_ERR_ARG, "nw_adjoint: null argument");
+  if (n_begin < 0 || n_end < n_begin || n_end > P->d.n_steps)
+    return fail(NW_ERR_ARG, "nw_adjoint: step range outside [0, n_steps]");
+  if (P->d.absorbing == NW_PML && !psi_bar) return fail(NW_ERR_ARG, "nw_adjoint: PML needs psi_bar");
+  if (keep && keep_begin > n_begin) return fail(NW_ERR_ARG, "nw_adjoint: keep must start <= n_begin");
+  if (grad_m && !keep) return fail(NW_ERR_ARG, "nw_adjoint: grad_m needs the kept wavefields");
+  hipStream_t st = (hipStream_t)stream;
+  if (P->d.dtype == NW_F32)
+    return adjoint_impl<float>(P, (const float*)m, (const float*)ma, lam, psi_bar, n_begin, n_end,
+                               (const float*)keep, keep_begin, (const float*)residual, inject_end,
+                               grad_m, first_step, st);
+  return adjoint_impl<double>(P, (const double*)m, (const double*)ma, lam, psi_bar, n_begin,
+                              n_end, (const double*)keep, keep_begin, (const double*)residual,
+                              inject_end, grad_m, first_step, st);
+}
+
+int32_t nw_model_grad(nw_plan* P, const void* c, const double* grad_m, void* grad_c, void* stream) {
+  if (!P || !c || !grad_m || !grad_c) return fail(NW_ERR_ARG, "nw_model_grad: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nf = P->d.field_dims[0] * P->d.field_dims[1] * P->d.field_dims[2];
+  const dim3 grd((unsigned)((nf + 255) / 256));
+  if (P->d.dtype == NW_F32)
+    hipLaunchKernelGGL(k_model_grad<float>, grd, dim3(256), 0, st, P->gf, (const float*)c, grad_m,
+                       (float*)grad_c, (int64_t)P->d.n_shots);
+  else
+    hipLaunchKernelGGL(k_model_grad<double>, grd, dim3(256), 0, st, P->gd, (const double*)c,
+                       grad_m, (double*)grad_c, (int64_t)P->d.n_shots);
+  return check_launch("k_model_grad");
+}
+
+int32_t nw_ic_state(nw_plan* P, const void* m, const void* p0, void* p_prev, void* p, void* stream) {
+  if (!P || !m || !p0 || !p_prev || !p) return fail(NW_ERR_ARG, "nw_ic_state: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* D = P->d.dims;
+  const dim3 blk = block_for(D);
+  const dim3 grd = grid_for(D, P->d.n_shots, blk);
+  const bool o4 = P->d.order == 4;
+  if (P->d.dtype == NW_F32) {
+    if (o4) hipLaunchKernelGGL((k_ic_state<float, 2>), grd, blk, 0, st, P->gf, (const float*)m, (const float*)p0, (float*)p_prev, (float*)p);
+    else hipLaunchKernelGGL((k_ic_state<float, 1>), grd, blk, 0, st, P->gf, (const float*)m, (const float*)p0, (float*)p_prev, (float*)p);
+  } else {
+    if (o4) hipLaunchKernelGGL((k_ic_state<double, 2>), grd, blk, 0, st, P->gd, (const double*)m, (const double*)p0, (double*)p_prev, (double*)p);
+    else hipLaunchKernelGGL((k_ic_state<double, 1>), grd, blk, 0, st, P->gd, (const double*)m, (const double*)p0, (double*)p_prev, (double*)p);
+  }
+  return check_launch("k_ic_state");
+}
+
+int32_t nw_ic_grad(nw_plan* P, const void* m, const void* lam0, const void* pbar, void* grad_p0,
+                   void* stream) {
+  if (!P || !m || !lam0 || !pbar || !grad_p0) return fail(NW_ERR_ARG, "nw_ic_grad: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* D = P->d.dims;
+  const dim3 blk = block_for(D);
+  const dim3 grd = grid_for(D, P->d.n_shots, blk);
+  const bool o4 = P->d.order == 4;
+  if (P->d.dtype == NW_F32) {
+    if (o4) hipLaunchKernelGGL((k_ic_grad<float, 2>), grd, blk, 0, st, P->gf, (const float*)m, (const float*)lam0, (const float*)pbar, (float*)grad_p0);
+    else hipLaunchKernelGGL((k_ic_grad<float, 1>), grd, blk, 0, st, P->gf, (const float*)m, (const float*)lam0, (const float*)pbar, (float*)grad_p0);
+  } else {
+    if (o4) hipLaunchKernelGGL((k_ic_grad<double, 2>), grd, blk, 0, st, P->gd, (const double*)m, (const double*)lam0, (const double*)pbar, (double*)grad_p0);
+    else hipLaunchKernelGGL((k_ic_grad<double, 1>), grd, blk, 0, st, P->gd, (const double*)m, (const double*)lam0, (const double*)pbar, (double*)grad_p0);
+  }
+  return check_launch("k_ic_grad");
+}
+
+int32_t nw_prev_adjoint(nw_plan* P, const void* lam1, void* pbar, void* stream) {
+  if (!P || !lam1 || !pbar) return fail(NW_ERR_ARG, "nw_prev_adjoint: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* D = P->d.dims;
+  const dim3 blk = block_for(D);
+  const dim3 grd = grid_for(D, P->d.n_shots, blk);
+#define NW_PA(T, A) \
+  hipLaunchKernelGGL((k_prev_adjoint<T, A>), grd, blk, 0, st, geo<T>(P), (const T*)lam1, (T*)pbar)
+  if (P->d.dtype == NW_F32) {
+    switch (P->d.absorbing) {
+      case NW_NONE: NW_PA(float, NW_NONE); break;
+      case NW_SPONGE: NW_PA(float, NW_SPONGE); break;
+      case NW_CERJAN: NW_PA(float, NW_CERJAN); break;
+      default: NW_PA(float, NW_PML); break;
+    }
+  } else {
+    switch (P->d.absorbing) {
+      case NW_NONE: NW_PA(double, NW_NONE); break;
+      case NW_SPONGE: NW_PA(double, NW_SPONGE); break;
+      case NW_CERJAN: NW_PA(double, NW_CERJAN); break;
+      default: NW_PA(double, NW_PML); break;
+    }
+  }
+#undef NW_PA
+  return check_launch("k_prev_adjoint");
+}
+
+int32_t nw_laplacian(nw_plan* P, const void* p, void* out, int64_t count, void* stream) {
+  if (!P || !p || !out || count < 1) return fail(NW_ERR_ARG, "nw_laplacian: bad argument");
+  if (P->d.dims[0] * count > 65535) return fail(NW_ERR_ARG, "nw_laplacian: too many grids");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* D = P->d.dims;
+  const dim3 blk = block_for(D);
+  const dim3 grd = grid_for(D, count, blk);
+  const bool o4 = P->d.order == 4;
+  if (P->d.dtype == NW_F32) {
+    if (o4) hipLaunchKernelGGL((k_laplacian<float, 2>), grd, blk, 0, st, P->gf, (const float*)p, (float*)out);
+    else hipLaunchKernelGGL((k_laplacian<float, 1>), grd, blk, 0, st, P->gf, (const float*)p, (float*)out);
+  } else {
+    if (o4) hipLaunchKernelGGL((k_laplacian<double, 2>), grd, blk, 0, st, P->gd, (const double*)p, (double*)out);
+    else hipLaunchKernelGGL((k_laplacian<double, 1>), grd, blk, 0, st, P->gd, (const double*)p, (double*)out);
+  }
+  return check_launch("k_laplacian");
+}
+
+int32_t nw_clamp(nw_plan* P, void* p, const int32_t* cells, const void* values, int32_t n_cells,
+                 void* stream) {
+  if (!P || !p || !cells || !values || n_cells < 0) return fail(NW_ERR_ARG, "nw_clamp: bad argument");
+  if (n_cells == 0) return NW_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t cellsz = P->d.dims[0] * P->d.dims[1] * P->d.dims[2];
+  const dim3 grd((n_cells + 255) / 256, P->d.n_shots);
+  if (P->d.dtype == NW_F32)
+    hipLaunchKernelGGL(k_clamp<float>, grd, dim3(256), 0, st, cellsz, (float*)p, cells, (const float*)values, n_cells);
+  else
+    hipLaunchKernelGGL(k_clamp<double>, grd, dim3(256), 0, st, cellsz, (double*)p, cells, (const double*)values, n_cells);
+  return check_launch("k_clamp");
+}
+
+}  // extern "C"

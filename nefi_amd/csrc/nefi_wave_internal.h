@@ -1,0 +1,94 @@
+// Kernel parameter blocks shared by the generic and the 2-D kernels of libnefi_wave.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+namespace nw {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+
+// Geometry and constants of a plan, passed by value to every kernel.
+template <typename T>
+struct Geo {
+  int64_t D[3];       // padded extents, axis 2 fastest
+  int64_t F[3];       // field (unpadded) extents
+  int64_t nabs[3];    // absorbing cells per side
+  int64_t cells;      // D0*D1*D2
+  int64_t fcells[3];  // cells of the face grid of each axis
+  int used[3];        // axis carries a stencil
+  int nd;             // physical dimensions
+  T cc;               // centre coefficient: sum over used axes of c0 / h^2
+  T c1[3], c2[3];     // neighbour coefficients per axis (radius 1, 2)
+  T inv2h[3];         // 1 / (2 h) per axis (PML divergence)
+  T dt2;              // dt^2 in T
+  double dt2_d;       // dt^2 in double
+  const T* damp[3];   // see nw_desc.damp
+  const T* alpha[3];
+  const T* kappa[3];
+  const T* zf[3];
+  const T* zc[3];
+};
+
+template <typename T>
+struct StepIO {
+  const T* m;
+  const T* p_prev;
+  const T* p;
+  T* p_next;
+  const T* psi_old[3];
+  T* psi_new[3];
+  int taper_prev;      // Cerjan: taper p^{n-1} when reading it
+  int src_taps;
+  const int32_t* src_idx;
+  const T* src_val;
+  const T* wavelet;
+  int n;               // step index (wavelet sample)
+  T* ck_prev;          // checkpoint slot (or null)
+  T* ck_cur;
+  T* ck_psi[3];
+};
+
+template <typename T>
+struct SampleIO {
+  const T* p;
+  T* traces;
+  int j, n_t, taps;
+  const int32_t* rec_first;
+  const int32_t* rec_count;
+  const int32_t* trace_first;
+  const int32_t* rec_idx;
+  const T* rec_w;
+};
+
+template <typename T>
+struct InjectIO {
+  T* lam;
+  const T* residual;
+  int j, n_t;
+  const int32_t* inj_first;
+  const int32_t* inj_count;
+  const int32_t* inj_cell;
+  const int32_t* inj_off;
+  const int32_t* inj_rec;
+  const T* inj_w;
+  const int32_t* trace_first;
+};
+
+template <typename T>
+struct AdjIO {
+  const T* ma;         // m * a_eff
+  const T* lam1;       // lam^{n+1}
+  const T* lam2;       // lam^{n+2}
+  T* lam0;             // lam^n (may alias lam2: read pointwise before the write)
+  int taper_next;      // Cerjan: step n+1 tapers its p^{n}
+  const T* p_n;        // p^n for the imaging term (or null)
+  const T* psi_low[3];   // psi^{n-1/2} (PML imaging)
+  const T* psi_half[3];  // psi^{n+1/2}
+  const T* psib_in[3];   // adjoint of psi^{n+1/2}
+  T* psib_out[3];        // adjoint of psi^{n-1/2}
+  double* grad;        // per-shot accumulators (or null)
+};
+
+}  // namespace nw

@@ -1,0 +1,415 @@
+"""Device plans and the autograd Functions of the wave operators.
+
+A :class:`HostPlan` holds what ``_LeapfrogBase._static`` (nefi/physics/wave.py:459-504) and
+``WaveOperator._extra_static`` (wave.py:670-674) derive on the host, in float64, as 1-D
+profiles and tap lists. A :class:`DevicePlan` uploads it once per (device, dtype) and owns the
+``nw_plan`` of ``libnefi_wave.so``. :class:`Propagator` runs forward and reverse sweeps through
+the library; the two ``autograd.Function`` classes put the hand-written adjoint behind
+``torch.autograd`` in place of autograd's taped per-step graph (precedent for the convention:
+``nefi/operators/pde/adjoint.py:280-370``: forward under no_grad, ``once_differentiable``
+backward).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import torch
+from torch.autograd.function import once_differentiable
+
+from .. import _native as nat
+from ..errors import ConfigError
+
+
+@dataclass
+class HostPlan:
+    """Host-side float64 description of one simulation batch (see ``struct nw_desc``)."""
+
+    ndim: int
+    order: int
+    absorbing: int
+    dims: tuple[int, int, int]
+    n_abs: tuple[int, int, int]
+    field_dims: tuple[int, int, int]
+    h: tuple[float, float, float]
+    dt: float
+    n_steps: int
+    n_t: int
+    substeps: int
+    n_shots: int
+    damp: list = field(default_factory=lambda: [None, None, None])
+    pml_alpha: list = field(default_factory=lambda: [None, None, None])
+    pml_kappa: list = field(default_factory=lambda: [None, None, None])
+    pml_zf: list = field(default_factory=lambda: [None, None, None])
+    pml_zc: list = field(default_factory=lambda: [None, None, None])
+    src_idx: torch.Tensor | None = None  # (S, taps) int64
+    src_val: torch.Tensor | None = None  # (S, taps) f64
+    wavelet: torch.Tensor | None = None  # (n_steps,) f64
+    rec_first: torch.Tensor | None = None
+    rec_count: torch.Tensor | None = None
+    trace_first: torch.Tensor | None = None
+    rec_idx: torch.Tensor | None = None  # (rows, taps) int64
+    rec_w: torch.Tensor | None = None  # (rows, taps) f64
+    n_trace_rows: int = 0
+
+    @property
+    def cells(self) -> int:
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    @property
+    def used_axes(self) -> list[int]:
+        return list(range(3 - self.ndim, 3))
+
+
+def injection_tables(rec_idx: torch.Tensor, rec_w: torch.Tensor, groups):
+    """Transpose of receiver stencils grouped by distinct cell, per receiver group.
+
+    ``groups`` lists ``(first_row, count)`` of the receiver rows each shot samples; shots with
+    the same group share the tables. Returns ``(first, count, cell, off, rec, w)`` as in
+    ``nw_desc.inj_*``, entries ordered by (cell, receiver, tap).
+    """
+    cache: dict = {}
+    cells, offs, recs, ws = [], [0], [], []
+    first, count = [], []
+    for g in groups:
+        if g not in cache:
+            r0, nr = g
+            start = len(cells)
+            if nr > 0:
+                idx = rec_idx[r0 : r0 + nr].reshape(-1)
+                w = rec_w[r0 : r0 + nr].reshape(-1)
+                taps = rec_idx.shape[1]
+                rr = torch.arange(nr, dtype=torch.int64).repeat_interleave(taps)
+                order = torch.argsort(idx * (nr + 1) + rr, stable=True)
+                idx, w, rr = idx[order], w[order], rr[order]
+                uniq, counts = torch.unique_consecutive(idx, return_counts=True)
+                for u, n in zip(uniq.tolist(), counts.tolist()):
+                    cells.append(u)
+                    offs.append(offs[-1] + n)
+                recs.extend(rr.tolist())
+                ws.extend(w.tolist())
+            cache[g] = (start, len(cells) - start)
+        f, n = cache[g]
+        first.append(f)
+        count.append(n)
+    i32 = torch.int32
+    return (
+        torch.tensor(first, dtype=i32),
+        torch.tensor(count, dtype=i32),
+        torch.tensor(cells, dtype=i32),
+        torch.tensor(offs, dtype=i32),
+        torch.tensor(recs, dtype=i32),
+        torch.tensor(ws, dtype=torch.float64),
+    )
+
+
+class DevicePlan:
+    """One ``nw_plan`` plus the device copies of its tables (kept alive with the plan)."""
+
+    def __init__(self, host: HostPlan, device: torch.device, dtype: torch.dtype) -> None:
+        if dtype not in (torch.float32, torch.float64):
+            raise ConfigError(f"wave kernels compute in float32 or float64, got {dtype}")
+        lib = nat.lib()
+        self.host, self.device, self.dtype = host, device, dtype
+        self._hold: list[torch.Tensor] = []
+
+        def dev(t, dt=None):
+            if t is None:
+                return None
+            x = t.to(device=device, dtype=dt or dtype).contiguous()
+            self._hold.append(x)
+            return x.data_ptr()
+
+        d = nat.NwDesc()
+        d.ndim, d.order, d.absorbing = host.ndim, host.order, host.absorbing
+        d.dtype = nat.NW_F64 if dtype == torch.float64 else nat.NW_F32
+        for a in range(3):
+            d.dims[a], d.n_abs[a], d.field_dims[a] = host.dims[a], host.n_abs[a], host.field_dims[a]
+            d.h[a] = host.h[a]
+            d.damp[a] = dev(host.damp[a])
+            d.pml_alpha[a] = dev(host.pml_alpha[a])
+            d.pml_kappa[a] = dev(host.pml_kappa[a])
+            d.pml_zf[a] = dev(host.pml_zf[a])
+            d.pml_zc[a] = dev(host.pml_zc[a])
+        d.dt = host.dt
+        d.n_steps, d.n_t, d.substeps, d.n_shots = host.n_steps, host.n_t, host.substeps, host.n_shots
+        if host.src_idx is not None and host.src_idx.numel():
+            d.src_taps = host.src_idx.shape[1]
+            d.src_idx = dev(host.src_idx, torch.int32)
+            d.src_val = dev(host.src_val)
+            d.wavelet = dev(host.wavelet)
+        d.rec_taps = 1 << host.ndim
+        d.n_trace_rows = host.n_trace_rows
+        if host.rec_idx is not None and host.rec_idx.numel():
+            d.rec_max = int(host.rec_count.max())
+            d.rec_first = dev(host.rec_first, torch.int32)
+            d.rec_count = dev(host.rec_count, torch.int32)
+            d.trace_first = dev(host.trace_first, torch.int32)
+            d.rec_idx = dev(host.rec_idx, torch.int32)
+            d.rec_w = dev(host.rec_w)
+            groups = list(zip(host.rec_first.tolist(), host.rec_count.tolist()))
+            f, n, cell, off, rec, w = injection_tables(host.rec_idx, host.rec_w, groups)
+            d.inj_max = int(n.max()) if n.numel() else 0
+            d.inj_first, d.inj_count = dev(f, torch.int32), dev(n, torch.int32)
+            d.inj_cell, d.inj_off = dev(cell, torch.int32), dev(off, torch.int32)
+            d.inj_rec, d.inj_w = dev(rec, torch.int32), dev(w)
+        self.desc = d
+        handle = ctypes.c_void_p()
+        nat.check(lib.nw_plan_create(ctypes.byref(d), ctypes.byref(handle)), "nw_plan_create")
+        self.handle = handle
+        self.esize = 8 if dtype == torch.float64 else 4
+        self.ckpt_slot = lib.nw_ckpt_slot_bytes(handle) // self.esize
+        self.keep_slot = lib.nw_keep_slot_bytes(handle) // self.esize
+        S, cells = host.n_shots, host.cells
+        self.shot_numel = S * cells
+        self.face_numel = []
+        for a in host.used_axes if host.absorbing == nat.NW_PML else []:
+            n = S
+            for b in range(3):
+                n *= host.dims[b] + (b == a)
+            self.face_numel.append(n)
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order varies
+        h = getattr(self, "handle", None)
+        if h is not None and nat._LIB is not None:
+            nat._LIB.nw_plan_destroy(h)
+            self.handle = None
+
+    # ---- helpers ---------------------------------------------------------------------------
+    @property
+    def pml(self) -> bool:
+        return self.host.absorbing == nat.NW_PML
+
+    def stream(self) -> int:
+        return nat.stream_of(self.device)
+
+    def zeros(self, n: int, dtype=None) -> torch.Tensor:
+        return torch.zeros(n, device=self.device, dtype=dtype or self.dtype)
+
+    def empty(self, n: int, dtype=None) -> torch.Tensor:
+        return torch.empty(n, device=self.device, dtype=dtype or self.dtype)
+
+    def new_state(self) -> tuple[list[torch.Tensor], list[torch.Tensor]]:
+        state = [self.zeros(self.shot_numel), self.zeros(self.shot_numel)]
+        psi = [self.zeros(n) for n in self.face_numel] + [self.zeros(n) for n in self.face_numel]
+        return state, psi
+
+    @staticmethod
+    def _reorder(tensors: list[torch.Tensor], ptrs) -> list[torch.Tensor]:
+        by_ptr = {t.data_ptr(): t for t in tensors}
+        return [by_ptr[p] for p in ptrs]
+
+    # ---- library calls ---------------------------------------------------------------------
+    def model(self, c: torch.Tensor, with_ma: bool) -> tuple[torch.Tensor, torch.Tensor | None]:
+        cells = self.host.cells
+        m = self.empty(cells)
+        ma = self.empty(cells) if with_ma else None
+        nat.check(
+            nat.lib().nw_model(self.handle, c.data_ptr(), m.data_ptr(), nat.ptr(ma), self.stream()),
+            "nw_model",
+        )
+        return m, ma
+
+    def forward(self, m, state, psi, n0, n1, traces=None, ckpt=None, ckpt_every=0, keep=None,
+                first_step=0):
+        """Advance ``state``/``psi`` (lists, updated in place) over steps ``[n0, n1)``."""
+        sp = nat.ptr_array([state[0].data_ptr(), state[1].data_ptr()])
+        pp = nat.ptr_array([t.data_ptr() for t in psi]) if self.pml else None
+        rc = nat.lib().nw_forward(
+            self.handle, m.data_ptr(), sp, pp, n0, n1, nat.ptr(traces), nat.ptr(ckpt),
+            ckpt_every, nat.ptr(keep), n0, first_step, self.stream(),
+        )
+        nat.check(rc, "nw_forward")
+        state[:] = self._reorder(state, [sp[0], sp[1]])
+        if self.pml:
+            k = len(self.face_numel)
+            new = []
+            for a in range(k):
+                pair = [psi[a], psi[k + a]]
+                new.append(self._reorder(pair, [pp[a], pp[k + a]]))
+            psi[:] = [p[0] for p in new] + [p[1] for p in new]
+
+    def adjoint(self, ma, lam, psib, n0, n1, keep=None, residual=None, inject_end=False,
+                grad=None, first_step=0):
+        lp = nat.ptr_array([lam[0].data_ptr(), lam[1].data_ptr()])
+        pp = nat.ptr_array([t.data_ptr() for t in psib]) if self.pml else None
+        rc = nat.lib().nw_adjoint(
+            self.handle, ma.data_ptr(), ma.data_ptr(), lp, pp, n0, n1, nat.ptr(keep), n0,
+            nat.ptr(residual), int(inject_end), nat.ptr(grad), first_step, self.stream(),
+        )
+        nat.check(rc, "nw_adjoint")
+        lam[:] = self._reorder(lam, [lp[0], lp[1]])
+        if self.pml:
+            k = len(self.face_numel)
+            new = []
+            for a in range(k):
+                new.append(self._reorder([psib[a], psib[k + a]], [pp[a], pp[k + a]]))
+            psib[:] = [p[0] for p in new] + [p[1] for p in new]
+
+    def ckpt_views(self, ckpt: torch.Tensor, slot: int):
+        base = slot * self.ckpt_slot
+        n = self.shot_numel
+        prev = ckpt[base : base + n]
+        cur = ckpt[base + n : base + 2 * n]
+        faces, off = [], base + 2 * n
+        for fn in self.face_numel:
+            faces.append(ckpt[off : off + fn])
+            off += fn
+        return prev, cur, faces
+
+
+def grad_block(plan: DevicePlan, n_steps: int, mode: str, every: int | None) -> int:
+    """Snapshot interval K of the reverse sweep.
+
+    ``autograd`` keeps every step of the forward sweep (no recomputation); ``checkpoint`` and
+    ``adjoint`` keep the state every K steps (``checkpoint_every`` or ``ceil(sqrt(n_steps))``)
+    and recompute one block at a time. A keep buffer that would not fit in free device memory
+    falls back to ``ceil(sqrt(n_steps))``.
+    """
+    n = max(1, n_steps)
+    k = n if mode == "autograd" else int(every or math.ceil(math.sqrt(n)))
+    k = max(1, min(k, n))
+    if plan.device.type == "cuda":
+        free, _ = torch.cuda.mem_get_info(plan.device)
+        need = (k + 1) * plan.keep_slot * plan.esize
+        if need > 0.6 * free:
+            k = max(1, min(k, math.ceil(math.sqrt(n))))
+    return k
+
+
+class Propagator:
+    """Forward / reverse sweeps of one :class:`DevicePlan`."""
+
+    def __init__(self, plan: DevicePlan, grad_mode: str, checkpoint_every: int | None) -> None:
+        self.plan = plan
+        self.grad_mode = grad_mode
+        self.checkpoint_every = checkpoint_every
+
+    # ---- FWI: c -> traces ------------------------------------------------------------------
+    def fwi_forward(self, c: torch.Tensor, need_grad: bool):
+        P = self.plan
+        h = P.host
+        m, ma = P.model(c, need_grad)
+        state, psi = P.new_state()
+        traces = P.zeros(h.n_trace_rows * h.n_t)
+        N = h.n_steps
+        if not need_grad or N == 0:
+            P.forward(m, state, psi, 0, N, traces=traces)
+            return traces, None
+        K = grad_block(P, N, self.grad_mode, self.checkpoint_every)
+        nb = -(-N // K)
+        last0 = (nb - 1) * K
+        ckpt = P.empty((nb - 1) * P.ckpt_slot) if nb > 1 else None
+        if last0 > 0:
+            P.forward(m, state, psi, 0, last0, traces=traces, ckpt=ckpt, ckpt_every=K)
+        keep = P.empty((K + 1) * P.keep_slot)
+        P.forward(m, state, psi, last0, N, traces=traces, keep=keep)
+        return traces, (m, ma, ckpt, keep, K, nb)
+
+    def fwi_backward(self, c: torch.Tensor, saved, residual: torch.Tensor) -> torch.Tensor:
+        P = self.plan
+        h = P.host
+        nf = h.field_dims[0] * h.field_dims[1] * h.field_dims[2]
+        grad_c = P.zeros(nf)
+        if saved is None:
+            return grad_c
+        m, ma, ckpt, keep, K, nb = saved
+        N = h.n_steps
+        lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
+        psib = [P.zeros(n) for n in P.face_numel] * 2
+        psib = [t.clone() for t in psib]
+        grad_m = P.zeros(P.shot_numel, torch.float64)
+        state, psi = P.new_state()
+        for b in range(nb - 1, -1, -1):
+            n0, n1 = b * K, min(N, (b + 1) * K)
+            if b < nb - 1:
+                prev, cur, faces = P.ckpt_views(ckpt, b)
+                state[0].copy_(prev)
+                state[1].copy_(cur)
+                for a, f in enumerate(faces):
+                    psi[a].copy_(f)
+                n_end = n1 if P.pml else max(n0, n1 - 1)
+                P.forward(m, state, psi, n0, n_end, keep=keep)
+            P.adjoint(ma, lam, psib, n0, n1, keep=keep, residual=residual,
+                      inject_end=(b == nb - 1), grad=grad_m)
+        nat.check(
+            nat.lib().nw_model_grad(P.handle, c.data_ptr(), grad_m.data_ptr(), grad_c.data_ptr(),
+                                    P.stream()),
+            "nw_model_grad",
+        )
+        return grad_c
+
+    # ---- PAT: p0 -> traces -----------------------------------------------------------------
+    def ic_forward(self, m: torch.Tensor, p0: torch.Tensor) -> torch.Tensor:
+        P = self.plan
+        h = P.host
+        state, psi = P.new_state()
+        nat.check(
+            nat.lib().nw_ic_state(P.handle, m.data_ptr(), p0.data_ptr(), state[0].data_ptr(),
+                                  state[1].data_ptr(), P.stream()),
+            "nw_ic_state",
+        )
+        traces = P.zeros(h.n_trace_rows * h.n_t)
+        P.forward(m, state, psi, 0, h.n_steps, traces=traces)
+        return traces
+
+    def ic_backward(self, m: torch.Tensor, ma: torch.Tensor, residual: torch.Tensor) -> torch.Tensor:
+        P = self.plan
+        h = P.host
+        lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
+        psib = [P.zeros(n) for n in P.face_numel] + [P.zeros(n) for n in P.face_numel]
+        P.adjoint(ma, lam, psib, 0, h.n_steps, residual=residual, inject_end=True)
+        pbar = P.empty(P.shot_numel)
+        nat.check(nat.lib().nw_prev_adjoint(P.handle, lam[1].data_ptr(), pbar.data_ptr(),
+                                            P.stream()), "nw_prev_adjoint")
+        nf = h.field_dims[0] * h.field_dims[1] * h.field_dims[2]
+        g = P.empty(h.n_shots * nf)
+        nat.check(nat.lib().nw_ic_grad(P.handle, m.data_ptr(), lam[0].data_ptr(), pbar.data_ptr(),
+                                       g.data_ptr(), P.stream()), "nw_ic_grad")
+        return g
+
+
+class FwiFunction(torch.autograd.Function):
+    """``c`` (field grid, on the compute device, plan dtype) → flat traces; hand adjoint."""
+
+    @staticmethod
+    def forward(ctx, c: torch.Tensor, prop: Propagator):
+        need = bool(ctx.needs_input_grad[0])
+        with torch.no_grad():
+            traces, saved = prop.fwi_forward(c.detach().contiguous(), need)
+        ctx.prop = prop
+        ctx.saved = saved
+        ctx.save_for_backward(c)
+        return traces
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g: torch.Tensor):
+        (c,) = ctx.saved_tensors
+        prop = ctx.prop
+        res = g.to(device=prop.plan.device, dtype=prop.plan.dtype).contiguous()
+        grad = prop.fwi_backward(c.detach().contiguous(), ctx.saved, res)
+        ctx.saved = None
+        return grad.view_as(c), None
+
+
+class IcFunction(torch.autograd.Function):
+    """``p0`` (batch, field grid) → flat traces of the initial-value operator; hand adjoint."""
+
+    @staticmethod
+    def forward(ctx, p0: torch.Tensor, prop: Propagator, m: torch.Tensor, ma: torch.Tensor):
+        with torch.no_grad():
+            traces = prop.ic_forward(m, p0.detach().contiguous())
+        ctx.prop, ctx.m, ctx.ma = prop, m, ma
+        ctx.p0_shape = p0.shape
+        return traces
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g: torch.Tensor):
+        prop = ctx.prop
+        res = g.to(device=prop.plan.device, dtype=prop.plan.dtype).contiguous()
+        grad = prop.ic_backward(ctx.m, ctx.ma, res)
+        return grad.view(ctx.p0_shape), None, None, None
