@@ -1,0 +1,205 @@
+"""Benchmark: fused forward + adjoint FWI gradient of the 64-shot uneven 2-D survey.
+
+Workload (one "step" = one gradient evaluation dJ/dc: forward sweep with snapshots, block
+recomputation and the reverse sweep with the imaging sum): 1024² field grid, dx = 10 m,
+20-cell sponge (1064² padded), 4000 leapfrog steps at CFL 0.4, 64 shots at evenly spaced x and
+depths drawn from U[2, 40] cells, per-shot receiver counts drawn from U[64, 512] along the top
+row, snapshot every 200 steps. The sound speed is 2000 m/s + 1500 m/s·depth/L plus a smooth
+seeded perturbation (Gaussian-filtered N(0,1), σ = 10 cells, max 100 m/s); the residual fed to
+the reverse sweep is a fixed seeded random trace set. Inputs are resident on the GPU before
+timing starts.
+
+``value`` = padded cells × steps × shots / seconds per gradient (Gcell-updates/s).
+``roofline`` uses the algorithmic bytes per cell-update of the dominant kernel (16 B for the
+forward step: read p^n, p^{n-1}, m, write p^{n+1}; 28 B for the reverse step: 16 B adjoint
+stencil + 12 B imaging) over its average duration, timed with HIP events around every launch.
+``cpu_baseline`` times the FP64 oracle (same physics, C, OpenMP) on a bounded sample of the
+same workload on this machine's host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"step": 16.0, "adjoint": 28.0}
+
+
+def survey(n=1024, shots=64, n_t=4001, seed=0):
+    """Geometry and model of the uneven 64-shot survey (host, float64)."""
+    from scipy.ndimage import gaussian_filter
+
+    dx = 10.0
+    L = n * dx
+    rng = np.random.default_rng(seed)
+    z = (np.arange(n) + 0.5) / n
+    pert = gaussian_filter(rng.standard_normal((n, n)), 10.0, mode="wrap")
+    c = 2000.0 + 1500.0 * z[None, :] + 100.0 * pert / np.abs(pert).max()
+    c_max = 3600.0
+    depths = rng.uniform(2.0, 40.0, shots) * dx
+    counts = rng.integers(64, 513, shots)
+    shot_geo = []
+    for s in range(shots):
+        src = [[(s + 0.5) * L / shots, float(depths[s])]]
+        r = int(counts[s])
+        rec = [[(i + 0.5) * L / r, 0.5 * dx] for i in range(r)]
+        shot_geo.append((src, rec))
+    kw = dict(n_t=n_t, dt_obs=0.4 * dx / c_max, f0=15.0, c_max=c_max, order=4,
+              absorbing="sponge", absorb_width=20 * dx)
+    return dict(n=n, dx=dx, L=L, c=c, shots=shot_geo, kw=kw, counts=counts)
+
+
+def cpu_baseline(wl, budget_s=15.0):
+    """FP64 oracle gradient on shot 0 of the same grid, step count scaled to ~budget_s."""
+    from oracle.leapfrog import Problem
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = wl["n"]
+    src, rec = wl["shots"][0]
+    kw = dict(wl["kw"])
+
+    def run(n_t):
+        kw["n_t"] = n_t
+        P = Problem((n, n), ((0.0, wl["L"]), (0.0, wl["L"])), rec, sources=src, **kw)
+        res = np.random.default_rng(1).standard_normal((1, len(rec), n_t))
+        t0 = time.perf_counter()
+        P.fwi_gradient(wl["c"], res)
+        return time.perf_counter() - t0, P.cells * P.n_steps
+
+    t, work = run(6)
+    steps = int(max(5, min(400, (budget_s / max(t, 1e-3)) * 5)))
+    t, work = run(steps + 1)
+    return {
+        "value": work / t / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle gradient (forward storing every state + reverse sweep), 1 shot, "
+                  f"{steps} steps on the 1064^2 padded grid, {t:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=4001)
+    ap.add_argument("--ckpt", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the propagator on a single MI355X "
+              "(shots would shard across GPUs with one gradient all-reduce; not measured here)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from nefi_amd import _native as nat
+    from nefi_amd.domain import Domain
+    from nefi_amd.physics.batched import ShotBatchWaveOperator
+
+    dev = torch.device("cuda", 0)
+    wl = survey(args.n, args.shots, args.nt)
+    n = wl["n"]
+    dom = Domain((n, n), ((0.0, wl["L"]), (0.0, wl["L"])))
+    op = ShotBatchWaveOperator(dom, wl["shots"], grad_mode="checkpoint",
+                               checkpoint_every=args.ckpt, **wl["kw"])
+    c = torch.tensor(wl["c"], dtype=torch.float32, device=dev, requires_grad=True)
+    rows = int(sum(wl["counts"]))
+    residual = torch.randn(rows, op.n_t, device=dev, generator=torch.Generator(dev).manual_seed(1))
+    padded = op.grid.padded_shape
+    updates = padded[0] * padded[1] * op.n_steps * args.shots
+
+    def step():
+        c.grad = None
+        y = op({"c": c})
+        y.backward(residual)
+
+    print(f"[bench] {args.shots} shots, {padded[0]}x{padded[1]} padded, {op.n_steps} steps, "
+          f"K={args.ckpt}, receivers {rows}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+    nat.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        print(f"[bench] step {i} queued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kinds = {"step": nat.K_STEP, "adjoint": nat.K_ADJOINT, "sample": nat.K_SAMPLE,
+             "inject": nat.K_INJECT}
+    timing = {k: nat.profile_read(v) for k, v in kinds.items()}
+    nat.profile_enable(False)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = updates / (elapsed / args.steps) / 1e9
+
+    dom_kind = max(("step", "adjoint"), key=lambda k: timing[k][0])
+    tot_ms, launches = timing[dom_kind]
+    avg_s = tot_ms / 1e3 / max(1, launches)
+    cells_per_launch = padded[0] * padded[1] * args.shots
+    achieved = BYTES_PER_UPDATE[dom_kind] * cells_per_launch / avg_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": {"step": "k_step<float,2,SPONGE>", "adjoint": "k_adjoint<float,2,SPONGE>"}[dom_kind],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_cell_update": BYTES_PER_UPDATE[dom_kind],
+        "avg_launch_us": round(avg_s * 1e6, 2),
+    }
+    kernel_ms = {k: {"total_ms": round(v[0] / args.steps, 2), "launches": v[1] // args.steps}
+                 for k, v in timing.items()}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
+    line = {
+        "metric": "fwd+adjoint Gcell-updates/s (64-shot 2-D FWI gradient)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2-D FWI gradient, {n}^2 grid (+20-cell sponge = {padded[0]}^2), "
+                        f"{args.shots} shots, src depth U[2,40] cells, receivers U[64,512]/shot, "
+                        f"{op.n_steps} steps, snapshot every {args.ckpt}",
+            "grid": list(padded),
+            "shots": args.shots,
+            "steps_per_gradient": op.n_steps,
+            "receivers": rows,
+            "cell_updates_per_step": updates,
+        },
+        "roofline": roofline,
+        "kernels_per_gradient": kernel_ms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -195,6 +195,16 @@ int32_t nw_laplacian(nw_plan* plan, const void* p, void* out, int64_t count, voi
 int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* values,
                  int32_t n_cells, void* stream);
 
+/* Kernel timing (no counterpart in the reference; measurement support for benchmarks).
+ * While enabled, every launch of the step, reverse-step, sample and inject kernels is bracketed
+ * by HIP events on its own stream. nw_profile_read waits for the recorded events and returns
+ * the summed duration (ms) and launch count of one kernel kind:
+ * 0 = forward step, 1 = reverse step, 2 = receiver sample, 3 = residual inject.
+ * Enabling clears earlier records. Process-wide; not thread-safe. */
+enum nw_kernel_kind { NW_K_STEP = 0, NW_K_ADJOINT = 1, NW_K_SAMPLE = 2, NW_K_INJECT = 3 };
+int32_t nw_profile_enable(int32_t on);
+int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

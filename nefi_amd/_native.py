@@ -42,6 +42,8 @@ EXPORTS = (
     "nw_prev_adjoint",
     "nw_laplacian",
     "nw_clamp",
+    "nw_profile_enable",
+    "nw_profile_read",
 )
 
 
@@ -112,6 +114,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_prev_adjoint": (i32, [vp, vp, vp, vp]),
         "nw_laplacian": (i32, [vp, vp, vp, i64, vp]),
         "nw_clamp": (i32, [vp, vp, vp, vp, i32, vp]),
+        "nw_profile_enable": (i32, [i32]),
+        "nw_profile_read": (i32, [i32, POINTER(c_double), POINTER(c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -144,6 +148,20 @@ def lib() -> ctypes.CDLL:
         )
     _LIB = handle
     return handle
+
+
+K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT = 0, 1, 2, 3
+
+
+def profile_enable(on: bool) -> None:
+    check(lib().nw_profile_enable(int(on)), "nw_profile_enable")
+
+
+def profile_read(kind: int) -> tuple[float, int]:
+    """Summed duration (ms) and launch count of one kernel kind since profiling was enabled."""
+    ms, n = c_double(), c_int64()
+    check(lib().nw_profile_read(kind, ctypes.byref(ms), ctypes.byref(n)), "nw_profile_read")
+    return ms.value, n.value
 
 
 def check(rc: int, what: str) -> None:

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "nefi_wave.h"
 #include "nefi_wave_internal.h"
@@ -29,6 +30,54 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+// ------------------------------------------------------------------------------------------
+// optional per-launch timing (nw_profile_enable / nw_profile_read)
+// ------------------------------------------------------------------------------------------
+struct ProfRec {
+  int kind;
+  hipEvent_t start, stop;
+};
+
+struct Profiler {
+  bool on = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  void recycle() {
+    for (auto& r : recs) {
+      pool.push_back(r.start);
+      pool.push_back(r.stop);
+    }
+    recs.clear();
+  }
+};
+
+static Profiler g_prof;
+
+// brackets one launch; returns the record index or -1 when timing is off
+static int prof_begin(int kind, hipStream_t st) {
+  if (!g_prof.on) return -1;
+  ProfRec r{kind, g_prof.get(), g_prof.get()};
+  if (!r.start || !r.stop) return -1;
+  hipEventRecord(r.start, st);
+  g_prof.recs.push_back(r);
+  return (int)g_prof.recs.size() - 1;
+}
+
+static void prof_end(int id, hipStream_t st) {
+  if (id >= 0) hipEventRecord(g_prof.recs[id].stop, st);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -486,7 +535,9 @@ static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t s
   io.rec_w = (const T*)d.rec_w;
   const int threads = d.rec_max * d.rec_taps;
   dim3 grid((threads + 255) / 256, d.n_shots);
+  const int pid = prof_begin(NW_K_SAMPLE, st);
   hipLaunchKernelGGL(k_sample<T>, grid, dim3(256), 0, st, geo<T>(P), io);
+  prof_end(pid, st);
   return check_launch("k_sample");
 }
 
@@ -507,7 +558,9 @@ static int launch_inject(nw_plan* P, T* lam, const T* residual, int j, hipStream
   io.inj_w = (const T*)d.inj_w;
   io.trace_first = d.trace_first;
   dim3 grid((d.inj_max + 255) / 256, d.n_shots);
+  const int pid = prof_begin(NW_K_INJECT, st);
   hipLaunchKernelGGL(k_inject<T>, grid, dim3(256), 0, st, geo<T>(P), io);
+  prof_end(pid, st);
   return check_launch("k_inject");
 }
 
@@ -646,7 +699,9 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
           q += P->face_bytes[a];
         }
     }
+    const int pid = prof_begin(NW_K_STEP, st);
     dispatch_step<T>(P, io, st);
+    prof_end(pid, st);
     NW_TRY(check_launch("k_step"));
     prev = cur;
     cur = next;
@@ -748,7 +803,9 @@ static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** 
       io.psib_in[a] = qa[a];
       io.psib_out[a] = qb[a];
     }
+    const int pid = prof_begin(NW_K_ADJOINT, st);
     dispatch_adj<T>(P, io, st);
+    prof_end(pid, st);
     NW_TRY(check_launch("k_adjoint"));
     T* t = L1;
     L1 = L2;  // now lam^n
@@ -784,6 +841,29 @@ extern "C" {
 int32_t nw_abi_version(void) { return NW_ABI_VERSION; }
 
 int64_t nw_desc_bytes(void) { return (int64_t)sizeof(nw_desc); }
+
+int32_t nw_profile_enable(int32_t on) {
+  g_prof.recycle();
+  g_prof.on = on != 0;
+  return NW_OK;
+}
+
+int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches) {
+  if (!total_ms || !launches) return fail(NW_ERR_ARG, "nw_profile_read: null argument");
+  double ms = 0.0;
+  int64_t n = 0;
+  for (auto& r : g_prof.recs) {
+    if (r.kind != kind) continue;
+    NW_HIP(hipEventSynchronize(r.stop));
+    float t = 0.f;
+    NW_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    ms += t;
+    ++n;
+  }
+  *total_ms = ms;
+  *launches = n;
+  return NW_OK;
+}
 
 const char* nw_last_error(void) { return nw::g_err.c_str(); }
 
