@@ -145,6 +145,20 @@ int32_t nw_forward(nw_plan* plan, const void* m, void** state, void** psi, int32
                    int32_t n_end, void* traces, void* ckpt, int32_t ckpt_every, void* keep,
                    int32_t keep_begin, int32_t first_step, void* stream);
 
+/* Tiles of the fast 2-D float32 kernels (no counterpart in the reference; set by the
+ * project's autotuner). kind 0 = forward step, 1 = reverse step. rows = rows of the strip one
+ * wavefront marches down (4, 8, 16, 32 or 64; 0 selects the generic kernels),
+ * shots_per_group = shots one workgroup advances in turn. Plans that the fast kernels cover
+ * (2-D, float32, sponge/Cerjan/none, padded width a multiple of 4) start at rows = 16, 1 shot;
+ * others report rows = 0 and refuse nonzero rows. */
+int32_t nw_plan_set_tiles(nw_plan* plan, int32_t kind, int32_t rows, int32_t shots_per_group);
+int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
+                          int32_t* shots_per_group);
+
+/* Number of float64 gradient accumulator grids nw_adjoint writes (grad_m holds
+ * nw_grad_slices * padded cells doubles; nw_model_grad sums all of them). */
+int64_t nw_grad_slices(const nw_plan* plan);
+
 /* Bytes of one checkpoint slot and of one keep slot. */
 int64_t nw_ckpt_slot_bytes(const nw_plan* plan);
 int64_t nw_keep_slot_bytes(const nw_plan* plan);
@@ -160,8 +174,8 @@ int64_t nw_keep_slot_bytes(const nw_plan* plan);
  * residual: (n_trace_rows, n_t) = dJ/dtraces; injected at every recorded n in
  *   [n_begin, n_end) (the caller injects n_end's sample before the first call via the same
  *   rule: pass inject_end = 1).
- * grad_m: if not NULL, per-shot float64 accumulators (n_shots, padded cells) receive
- *   sum_n a_eff*lam^{n+1} * lap(p^n).
+ * grad_m: if not NULL, float64 accumulators (nw_grad_slices, padded cells) receive
+ *   sum_n a_eff*lam^{n+1} * lap(p^n), one grid per shot group.
  * first_step: as in nw_forward. */
 int32_t nw_adjoint(nw_plan* plan, const void* m, const void* ma, void** lam, void** psi_bar,
                    int32_t n_begin, int32_t n_end, const void* keep, int32_t keep_begin,

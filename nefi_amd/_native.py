@@ -44,6 +44,9 @@ EXPORTS = (
     "nw_clamp",
     "nw_profile_enable",
     "nw_profile_read",
+    "nw_plan_set_tiles",
+    "nw_plan_get_tiles",
+    "nw_grad_slices",
 )
 
 
@@ -116,6 +119,9 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_clamp": (i32, [vp, vp, vp, vp, i32, vp]),
         "nw_profile_enable": (i32, [i32]),
         "nw_profile_read": (i32, [i32, POINTER(c_double), POINTER(c_int64)]),
+        "nw_plan_set_tiles": (i32, [vp, i32, i32, i32]),
+        "nw_plan_get_tiles": (i32, [vp, i32, POINTER(c_int32), POINTER(c_int32)]),
+        "nw_grad_slices": (i64, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nefi_wave.h"
@@ -474,6 +475,10 @@ struct nw_plan {
   nw_desc d;
   Geo<float> gf;
   Geo<double> gd;
+  // tiles of the fast 2-D kernels: rows per wavefront strip (0 = generic kernels) and shots
+  // advanced by one workgroup; [0] forward step, [1] reverse step
+  int rows[2];
+  int spw[2];
   int64_t shot_bytes;          // one wavefield (all shots)
   int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
 };
@@ -581,6 +586,12 @@ static void launch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
 
 template <typename T>
 static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (P->rows[0] > 0) {
+      fast2d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[0], P->spw[0], st);
+      return;
+    }
+  }
   const int r = P->d.order == 4 ? 2 : 1;
   switch (P->d.absorbing * 2 + (r - 1)) {
     case NW_NONE * 2 + 0: launch_step<T, 1, NW_NONE>(P, io, st); break;
@@ -596,6 +607,13 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
 
 template <typename T>
 static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (P->rows[1] > 0) {
+      fast2d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[1], P->spw[1],
+                     st);
+      return;
+    }
+  }
   const int r = P->d.order == 4 ? 2 : 1;
   switch (P->d.absorbing * 2 + (r - 1)) {
     case NW_NONE * 2 + 0: launch_adj<T, 1, NW_NONE>(P, io, st); break;
@@ -902,6 +920,11 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->d = d;
   p->gf = make_geo<float>(d);
   p->gd = make_geo<double>(d);
+  const bool fast = fast2d_supported(p->gf, d.absorbing, d.dtype);
+  for (int k = 0; k < 2; ++k) {
+    p->rows[k] = fast ? 16 : 0;
+    p->spw[k] = 1;
+  }
   const size_t es = d.dtype == NW_F64 ? 8 : 4;
   p->shot_bytes = cells * (int64_t)es * d.n_shots;
   for (int a = 0; a < 3; ++a)
@@ -914,6 +937,37 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
 
 int32_t nw_plan_destroy(nw_plan* plan) {
   delete plan;
+  return NW_OK;
+}
+
+int64_t nw_grad_slices(const nw_plan* P) {
+  if (P->d.dtype == NW_F32 && P->rows[1] > 0) return (P->d.n_shots + P->spw[1] - 1) / P->spw[1];
+  return P->d.n_shots;
+}
+
+int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_per_group) {
+  if (!P || kind < 0 || kind > 1) return fail(NW_ERR_ARG, "nw_plan_set_tiles: bad plan or kind");
+  if (rows == 0) {
+    P->rows[kind] = 0;
+    P->spw[kind] = 1;
+    return NW_OK;
+  }
+  if (!fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
+    return fail(NW_ERR_UNSUPPORTED, "nw_plan_set_tiles: the fast 2-D kernels do not cover this plan");
+  if (rows != 4 && rows != 8 && rows != 16 && rows != 32 && rows != 64)
+    return fail(NW_ERR_ARG, "nw_plan_set_tiles: rows must be 4, 8, 16, 32 or 64");
+  if (shots_per_group < 1 || shots_per_group > P->d.n_shots)
+    return fail(NW_ERR_ARG, "nw_plan_set_tiles: shots_per_group must be in [1, n_shots]");
+  P->rows[kind] = rows;
+  P->spw[kind] = shots_per_group;
+  return NW_OK;
+}
+
+int32_t nw_plan_get_tiles(const nw_plan* P, int32_t kind, int32_t* rows, int32_t* shots_per_group) {
+  if (!P || kind < 0 || kind > 1 || !rows || !shots_per_group)
+    return fail(NW_ERR_ARG, "nw_plan_get_tiles: bad argument");
+  *rows = P->rows[kind];
+  *shots_per_group = P->spw[kind];
   return NW_OK;
 }
 
@@ -994,12 +1048,13 @@ int32_t nw_model_grad(nw_plan* P, const void* c, const double* grad_m, void* gra
   hipStream_t st = (hipStream_t)stream;
   const int64_t nf = P->d.field_dims[0] * P->d.field_dims[1] * P->d.field_dims[2];
   const dim3 grd((unsigned)((nf + 255) / 256));
+  const int64_t slices = nw_grad_slices(P);
   if (P->d.dtype == NW_F32)
     hipLaunchKernelGGL(k_model_grad<float>, grd, dim3(256), 0, st, P->gf, (const float*)c, grad_m,
-                       (float*)grad_c, (int64_t)P->d.n_shots);
+                       (float*)grad_c, slices);
   else
     hipLaunchKernelGGL(k_model_grad<double>, grd, dim3(256), 0, st, P->gd, (const double*)c,
-                       grad_m, (double*)grad_c, (int64_t)P->d.n_shots);
+                       grad_m, (double*)grad_c, slices);
   return check_launch("k_model_grad");
 }
 

@@ -91,4 +91,11 @@ struct AdjIO {
   double* grad;        // per-shot accumulators (or null)
 };
 
+// fast 2-D float32 kernels (nefi_wave_2d.hip)
+bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype);
+void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
+                 int ty, int spw, void* stream);
+void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
+                    int shots, int ty, int spw, void* stream);
+
 }  // namespace nw

@@ -177,6 +177,20 @@ class DevicePlan:
             nat._LIB.nw_plan_destroy(h)
             self.handle = None
 
+    # ---- tiles of the fast 2-D kernels -----------------------------------------------------
+    def tiles(self, kind: int) -> tuple[int, int]:
+        rows, spw = ctypes.c_int32(), ctypes.c_int32()
+        nat.check(nat.lib().nw_plan_get_tiles(self.handle, kind, ctypes.byref(rows),
+                                              ctypes.byref(spw)), "nw_plan_get_tiles")
+        return rows.value, spw.value
+
+    def set_tiles(self, kind: int, rows: int, shots_per_group: int) -> None:
+        nat.check(nat.lib().nw_plan_set_tiles(self.handle, kind, rows, shots_per_group),
+                  "nw_plan_set_tiles")
+
+    def grad_slices(self) -> int:
+        return int(nat.lib().nw_grad_slices(self.handle))
+
     # ---- helpers ---------------------------------------------------------------------------
     @property
     def pml(self) -> bool:
@@ -320,7 +334,7 @@ class Propagator:
         lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
         psib = [P.zeros(n) for n in P.face_numel] * 2
         psib = [t.clone() for t in psib]
-        grad_m = P.zeros(P.shot_numel, torch.float64)
+        grad_m = P.zeros(P.grad_slices() * h.cells, torch.float64)
         state, psi = P.new_state()
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)

@@ -1,0 +1,113 @@
+"""Tile autotuner of the fast 2-D kernels.
+
+For a plan the fast kernels cover, time a few forward steps and a few reverse steps for every
+candidate tile — rows of the strip one wavefront marches down × shots one workgroup advances
+in turn — and keep the fastest per kernel. Timing uses the library's own per-launch HIP events
+(``nw_profile_*``), so it measures the kernels alone. Results are cached in a JSON file keyed by
+device name, padded shape, shot count, order and absorbing layer
+(``$NEFI_AMD_TILE_CACHE``, default ``~/.cache/nefi_amd/tiles.json``); ``NEFI_AMD_AUTOTUNE=0``
+keeps the library's defaults.
+
+This is the kernel autotuner the north star asks for; it is unrelated to nefi's
+hyper-parameter package ``nefi/autotune``.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import torch
+
+from .. import _native as nat
+
+ROWS = (4, 8, 16, 32, 64)
+GROUPS = (1, 2, 4, 8)
+# below this many cell-updates per step a launch is latency-bound and every tile ties
+MIN_WORK = 1 << 18
+
+
+def _cache_path() -> Path:
+    p = os.environ.get("NEFI_AMD_TILE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "nefi_amd" / "tiles.json"
+
+
+def _load() -> dict:
+    try:
+        return json.loads(_cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict) -> None:
+    path = _cache_path()
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        tmp.write_text(json.dumps(table, indent=1, sort_keys=True))
+        tmp.replace(path)
+    except OSError:
+        pass
+
+
+def plan_key(plan) -> str:
+    h = plan.host
+    name = torch.cuda.get_device_name(plan.device) if plan.device.type == "cuda" else "cpu"
+    return f"{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
+
+
+def candidates(n_shots: int):
+    return [(r, g) for r in ROWS for g in GROUPS if g <= n_shots]
+
+
+def _time_kind(plan, kind: int, rows: int, spw: int, steps: int) -> float:
+    """Average launch time (ms) of one kernel kind at one tile."""
+    P = plan
+    P.set_tiles(kind, rows, spw)
+    cells = P.host.cells
+    m = P.zeros(cells)
+    state, psi = P.new_state()
+    nat.profile_enable(True)
+    if kind == 0:
+        P.forward(m, state, psi, 0, steps)
+        k = nat.K_STEP
+    else:
+        keep = P.zeros((steps + 1) * P.keep_slot)
+        lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
+        grad = P.zeros(P.grad_slices() * cells, torch.float64)
+        P.adjoint(m, lam, psi, 0, steps, keep=keep, grad=grad)
+        k = nat.K_ADJOINT
+    ms, n = nat.profile_read(k)
+    nat.profile_enable(False)
+    return ms / max(1, n)
+
+
+def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
+    """Pick and set the fastest tiles of ``plan`` (a ``DevicePlan``); returns the choice."""
+    if os.environ.get("NEFI_AMD_AUTOTUNE", "1") == "0":
+        return None
+    if plan.tiles(0)[0] == 0:  # the fast kernels do not cover this plan
+        return None
+    h = plan.host
+    if h.cells * h.n_shots < MIN_WORK and not force:
+        return None
+    key = plan_key(plan)
+    table = _load()
+    if key in table and not force:
+        choice = table[key]
+    else:
+        steps = min(steps, max(1, h.n_steps))
+        choice = {}
+        for kind, name in ((0, "step"), (1, "adjoint")):
+            timings = {}
+            for rows, spw in candidates(h.n_shots):
+                _time_kind(plan, kind, rows, spw, 1)  # first launch of a tile compiles nothing
+                timings[(rows, spw)] = _time_kind(plan, kind, rows, spw, steps)
+            (rows, spw), best = min(timings.items(), key=lambda kv: kv[1])
+            choice[name] = {"rows": rows, "shots_per_group": spw, "ms": best}
+        table[key] = choice
+        _store(table)
+    plan.set_tiles(0, choice["step"]["rows"], choice["step"]["shots_per_group"])
+    plan.set_tiles(1, choice["adjoint"]["rows"], choice["adjoint"]["shots_per_group"])
+    return choice

@@ -33,6 +33,7 @@ from ..errors import ConfigError, OperatorError, ShapeError
 from ..operators.base import Fields, Operator
 from ..registry import register
 from ..utils import resample, shape_tuple
+from . import autotune
 from ._engine import DevicePlan, FwiFunction, HostPlan, IcFunction, Propagator
 from .timestep import auto_checkpoint_every, stable_dt_wave
 
@@ -496,6 +497,7 @@ class _LeapfrogBase(Operator):
         if prop is None:
             prop = Propagator(DevicePlan(make_host(), device, dtype), self.grad_mode,
                               self.checkpoint_every)
+            autotune.tune(prop.plan)
             self._cache[full] = prop
         prop.grad_mode = self.grad_mode  # instances switch it after construction
         prop.checkpoint_every = self.checkpoint_every

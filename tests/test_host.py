@@ -1,0 +1,223 @@
+"""Host-side logic of the package (CPU, no GPU): the C ABI surface, geometry and tables that
+the operators hand to the library, the reference's error behaviour, and the generic helpers of
+nefi/physics/timestep.py."""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from _support import load, oracle_problem
+from nefi_amd import _native as nat
+from nefi_amd.domain import Domain
+from nefi_amd.errors import ConfigError, NativeError, OperatorError, ShapeError
+from nefi_amd.physics import timestep as ts
+from nefi_amd.physics.wave import (
+    ABSORBING,
+    GRAD_MODES,
+    WaveGrid,
+    WaveInitialConditionOperator,
+    WaveOperator,
+    laplacian_kernel,
+    ricker,
+    stable_dt,
+)
+from nefi_amd.registry import build, get
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+# ---- C ABI ---------------------------------------------------------------------------------
+def test_library_loads_and_exports_every_declared_symbol():
+    header = (ROOT / "include" / "nefi_wave.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int32_t|int64_t|const char\*)\s+(nw_\w+)\(", header, re.M))
+    assert declared == set(nat.EXPORTS)
+    lib = nat.lib()
+    for name in declared:
+        assert getattr(lib, name) is not None
+    assert lib.nw_abi_version() == nat.ABI_VERSION
+    assert lib.nw_desc_bytes() == ctypes.sizeof(nat.NwDesc)
+
+
+def test_plan_create_rejects_bad_descriptors_without_a_gpu():
+    lib = nat.lib()
+    d = nat.NwDesc()
+    h = ctypes.c_void_p()
+    d.ndim = 4
+    assert lib.nw_plan_create(ctypes.byref(d), ctypes.byref(h)) == 1
+    assert b"ndim" in lib.nw_last_error()
+    d.ndim, d.order, d.n_shots, d.substeps, d.n_t = 2, 3, 1, 1, 1
+    assert lib.nw_plan_create(ctypes.byref(d), ctypes.byref(h)) == 1
+    assert b"order" in lib.nw_last_error()
+
+
+def test_operators_raise_without_gpu_instead_of_falling_back():
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    meta, z = load("pml2d")
+    op = WaveOperator(Domain(meta["shape"], meta["extent"]), meta["sources"], meta["receivers"],
+                      **meta["kw"])
+    with pytest.raises(NativeError, match="no CPU fallback"):
+        op({"c": torch.tensor(z["in_c"])})
+
+
+# ---- geometry against the independently written oracle geometry --------------------------
+@pytest.mark.parametrize("name", ["cfg1", "pml2d", "cerjan2d", "pml1d_o2", "pat3d"])
+def test_grid_and_stencils_match_oracle(name):
+    meta, z = load(name)
+    P = oracle_problem(meta, z)
+    op_kw = {k: v for k, v in meta["kw"].items()}
+    dom = Domain(meta["shape"], meta["extent"])
+    if meta["kind"] == "fwi":
+        op = WaveOperator(dom, meta["sources"], meta["receivers"], **op_kw)
+        si, sw = op.grid.point_weights(op.sources)
+        oi, ow = P.point_weights(np.asarray(meta["sources"]))
+        assert np.array_equal(si.numpy(), oi) and np.allclose(sw.numpy(), ow, rtol=0, atol=0)
+    else:
+        op_kw["c"] = torch.as_tensor(z["in_kw_c"]) if "in_kw_c" in z else op_kw["c"]
+        op = WaveInitialConditionOperator(dom, meta["receivers"], **op_kw)
+    g = op.grid
+    assert g.n_abs == P.n_abs and g.padded_shape == P.padded
+    assert op.substeps == P.substeps and op.n_steps == P.n_steps and op.dt == P.dt
+    ri, rw = op._rec
+    assert np.array_equal(ri.numpy(), P.rec_idx) and np.array_equal(rw.numpy(), P.rec_w)
+
+
+def test_point_weights_edge_clamp_and_outside_error():
+    g = WaveGrid(Domain((4, 5), ((0.0, 4.0), (0.0, 5.0))), absorb_width=1.0)
+    assert g.padded_shape == (6, 7) and g.lo_pad == (-1.0, -1.0)
+    # the last cell centre: i0 clamped to m-2, weights (0, 1)
+    idx, w = g.point_weights([[4.5, 5.5]])
+    assert idx.tolist() == [[4 * 7 + 5, 4 * 7 + 6, 5 * 7 + 5, 5 * 7 + 6]]
+    assert np.allclose(w.numpy(), [[0, 0, 0, 1]])
+    with pytest.raises(ConfigError, match="outside"):
+        g.point_weights([[-1.0, 0.0]])
+    assert g.nearest_index([[0.1, 0.1]]).tolist() == [1 * 7 + 1]
+
+
+def test_host_profiles_match_reference_formulas():
+    """The 1-D profiles handed to the kernels recombine into nefi's full-grid coefficients."""
+    dom = Domain((12, 10), ((0.0, 1.2), (0.0, 1.0)))
+    for absorbing in ("sponge", "pml", "cerjan"):
+        op = WaveOperator(dom, [[0.6, 0.5]], [[0.3, 0.4]], n_t=10, dt_obs=0.02, f0=5.0, c_max=2.0,
+                          absorbing=absorbing, absorb_width=0.3)
+        hp = op._host_plan(1)
+        P = oracle_problem({"kind": "fwi", "shape": (12, 10), "extent": ((0.0, 1.2), (0.0, 1.0)),
+                            "sources": [[0.6, 0.5]], "receivers": [[0.3, 0.4]],
+                            "kw": dict(n_t=10, dt_obs=0.02, f0=5.0, c_max=2.0,
+                                       absorbing=absorbing, absorb_width=0.3)}, None)
+        if absorbing in ("sponge", "pml"):
+            half = hp.damp[1][:, None] + hp.damp[2][None, :]
+            assert np.allclose((1.0 / (1.0 + half)).numpy(), P.a, rtol=1e-15)
+            assert np.allclose((1.0 - half).numpy(), P.b, rtol=1e-15)
+        else:
+            tap = torch.minimum(hp.damp[1][:, None], hp.damp[2][None, :])
+            assert np.array_equal(tap.numpy(), P.taper)
+        if absorbing == "pml":
+            zz = op.dt**2 * hp.pml_zc[1][:, None] * hp.pml_zc[2][None, :]
+            assert np.allclose(zz.numpy(), P.zz, rtol=1e-14, atol=0)
+            for k, ax in ((1, 0), (2, 1)):
+                other = hp.pml_zc[3 - k]
+                shape = [1, 1]
+                shape[1 - ax] = -1
+                beta = (other.view(shape) - hp.pml_zf[k].view([-1 if i == ax else 1
+                                                                 for i in range(2)])) \
+                    * hp.pml_kappa[k].view([-1 if i == ax else 1 for i in range(2)])
+                assert np.allclose(np.broadcast_to(beta.numpy(), P.beta[ax].shape), P.beta[ax],
+                                   rtol=1e-13, atol=1e-300)
+                alpha = hp.pml_alpha[k].view([-1 if i == ax else 1 for i in range(2)])
+                assert np.allclose(np.broadcast_to(alpha.numpy(), P.alpha[ax].shape),
+                                   P.alpha[ax], rtol=1e-15)
+
+
+def test_injection_tables_are_the_transpose_of_the_receiver_stencils():
+    from nefi_amd.physics._engine import injection_tables
+
+    idx = torch.tensor([[3, 4, 8, 9], [4, 5, 9, 10], [20, 21, 25, 26]])
+    w = torch.rand(3, 4, dtype=torch.float64)
+    first, count, cell, off, rec, ww = injection_tables(idx, w, [(0, 2), (2, 1), (0, 2)])
+    assert first.tolist() == [0, 6, 0] and count.tolist() == [6, 4, 6]
+    dense = np.zeros((2, 30))
+    for u in range(6):
+        for e in range(int(off[u]), int(off[u + 1])):
+            dense[int(rec[e]), int(cell[u])] += float(ww[e])
+    want = np.zeros((2, 30))
+    for r in range(2):
+        for k in range(4):
+            want[r, int(idx[r, k])] += float(w[r, k])
+    assert np.allclose(dense, want)
+
+
+# ---- API and errors (wave.py:369-378, 561-569, 711-712) -------------------------------------
+def test_api_constants_errors_and_registry():
+    assert ABSORBING == ("pml", "sponge", "cerjan", "none")
+    assert set(GRAD_MODES) >= {"checkpoint", "autograd"}
+    dom = Domain((16, 16), ((0.0, 1.0), (0.0, 1.0)))
+    kw = dict(n_t=20, dt_obs=0.01, f0=5.0, c_max=2.5)
+    with pytest.raises(ConfigError):
+        WaveOperator(dom, [[0.5, 0.5]], [[0.2, 0.2]], absorbing="foo", **kw)
+    with pytest.raises(ConfigError):
+        WaveOperator(dom, [[0.5, 0.5]], [[0.2, 0.2]], grad_mode="foo", **kw)
+    with pytest.raises(ConfigError):
+        WaveOperator(dom, [[0.5, 0.5]], [[0.2, 0.2]], absorb_R=1.5, **kw)
+    with pytest.raises(ConfigError):
+        WaveOperator(Domain((4, 4, 4), ((0, 1),) * 3), [[0.5] * 3], [[0.2] * 3], **kw)
+    with pytest.raises(ShapeError):
+        WaveOperator(dom, [[0.5, 0.5, 0.5]], [[0.2, 0.2]], **kw)
+    op = WaveOperator(dom, [[0.5, 0.5], [0.4, 0.4]], [[0.2, 0.2]] * 3, **kw)
+    assert op.output_shape((16, 16)) == (2, 3, 20)
+    assert op.fidelity_tag == "wave-leapfrog-o4-pml" and op.traceable is False
+    assert op.record == [j * op.substeps for j in range(20)]
+    coarse = op.at_resolution((8, 8))
+    assert coarse is op.at_resolution((8, 8)) and coarse.fidelity_tag == op.fidelity_tag
+    with pytest.raises(ShapeError):
+        op._prepare(torch.ones(8, 8), None)
+    with pytest.raises(OperatorError, match="exceeds"):
+        op._check_speed(torch.full((16, 16), 3.0))
+    with pytest.raises(OperatorError, match="positive"):
+        op._check_speed(torch.zeros(16, 16))
+    assert get("operator", "wave") is WaveOperator
+    assert isinstance(build("operator", {"type": "wave", "domain": dom, "sources": [[0.5, 0.5]],
+                                         "receivers": [[0.2, 0.2]], **kw}), WaveOperator)
+    ic = WaveInitialConditionOperator(dom, [[0.2, 0.2]], n_t=10, dt_obs=0.01, c=1.0)
+    assert ic.homogeneity == 1.0 and ic.output_shape((16, 16)) == (1, 10)
+
+
+def test_ricker_stable_dt_and_kernel():
+    t = torch.linspace(0, 1, 2001, dtype=torch.float64)
+    w = ricker(t, 5.0)
+    assert abs(float(w.max()) - 1.0) < 1e-6 and abs(float(t[w.argmax()]) - 0.24) < 1e-3
+    assert math.isclose(stable_dt(1.0, (0.1, 0.1), order=2, courant=1.0), 0.1 / math.sqrt(2))
+    assert math.isclose(stable_dt(1.0, (0.1, 0.1), order=4, courant=1.0), 0.1 * math.sqrt(3 / 8))
+    k = laplacian_kernel((0.5, 0.25), 4, dtype=torch.float64)[0, 0]
+    assert k.shape == (5, 5) and math.isclose(float(k[2, 2]), -2.5 / 0.25 - 2.5 / 0.0625)
+    assert math.isclose(float(k[0, 2]), -1 / 12 / 0.25) and math.isclose(float(k[2, 3]), 4 / 3 / 0.0625)
+
+
+# ---- generic time stepping (timestep.py:78-169) --------------------------------------------
+def test_run_timestepping_record_and_checkpoint_gradients():
+    k = torch.tensor([0.5, 1.0, 2.0], dtype=torch.float64, requires_grad=True)
+
+    def step(s, p, n, dt):
+        return (s[0] - dt * p[0] * s[0],)
+
+    u0 = torch.ones(3, dtype=torch.float64)
+    (uT,), obs = ts.run_timestepping(step, u0, (k,), 100, 0.01, lambda s, n: s[0].clone(),
+                                     record=[0, 50, 100])
+    assert len(obs) == 3 and torch.allclose(obs[0], u0)
+    assert torch.allclose(uT.detach(), (1 - 0.01 * k.detach()) ** 100, rtol=1e-12)
+    g1 = torch.autograd.grad(sum(o.sum() for o in obs), k)[0]
+    _, obs2 = ts.run_timestepping(step, u0, (k,), 100, 0.01, lambda s, n: s[0].clone(),
+                                  record=[0, 50, 100], checkpoint_every=7)
+    g2 = torch.autograd.grad(sum(o.sum() for o in obs2), k)[0]
+    assert torch.allclose(g1, g2, rtol=1e-12, atol=0)
+    assert ts.auto_checkpoint_every(100) == 10
+    with pytest.raises(ConfigError):
+        ts.run_timestepping(step, u0, (k,), 5, 0.1, record=[7])
+    assert math.isclose(ts.stable_dt_diffusion(1.0, (0.1, 0.1), safety=1.0), 0.1**2 / 4)
