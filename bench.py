@@ -100,6 +100,9 @@ def main() -> int:
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 512 MiB read + 512 MiB write elementwise kernel "
+                         "(reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the propagator on a single MI355X "
@@ -114,6 +117,13 @@ def main() -> int:
     from nefi_amd.physics.batched import ShotBatchWaveOperator
 
     dev = torch.device("cuda", 0)
+    if args.calibrate:
+        a = torch.ones(1 << 27, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     wl = survey(args.n, args.shots, args.nt)
     n = wl["n"]
     dom = Domain((n, n), ((0.0, wl["L"]), (0.0, wl["L"])))
@@ -151,14 +161,24 @@ def main() -> int:
     ms_per_step = elapsed * 1e3 / args.steps
     value = updates / (elapsed / args.steps) / 1e9
 
+    plan = next(iter(op._cache.values())).plan
+    tiles = {name: dict(zip(("rows", "shots_per_group"), plan.tiles(kind)))
+             for kind, name in ((0, "step"), (1, "adjoint"))}
+    tiles["xcd_map"] = plan.option(nat.OPT_XCD_MAP)
+    tiles["rows_per_iter"] = plan.option(nat.OPT_ROWS_PER_ITER)
     dom_kind = max(("step", "adjoint"), key=lambda k: timing[k][0])
+    rows_t = tiles[dom_kind]["rows"]
+    kname = {"step": "k2_step" if rows_t else "k_step",
+             "adjoint": "k2_adjoint" if rows_t else "k_adjoint"}[dom_kind]
+    kname += f"<order 4, sponge{', rows %d' % rows_t if rows_t else ''}>"
     tot_ms, launches = timing[dom_kind]
-    avg_s = tot_ms / 1e3 / max(1, launches)
+    # (under a counter-collecting profiler the library's events record nothing)
+    avg_s = tot_ms / 1e3 / launches if launches and tot_ms > 0 else float("nan")
     cells_per_launch = padded[0] * padded[1] * args.shots
-    achieved = BYTES_PER_UPDATE[dom_kind] * cells_per_launch / avg_s / 1e9
+    achieved = BYTES_PER_UPDATE[dom_kind] * cells_per_launch / avg_s / 1e9 if avg_s == avg_s else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": {"step": "k_step<float,2,SPONGE>", "adjoint": "k_adjoint<float,2,SPONGE>"}[dom_kind],
+        "kernel": kname,
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -195,6 +215,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "kernels_per_gradient": kernel_ms,
+        "tiles": tiles,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -13,6 +13,8 @@
  *    PyTorch). The library owns only the plan object.
  *  - Grids are row-major with axis 2 fastest. A d-dimensional padded grid (d = 1..3) is stored
  *    as dims = (1,..,1, n_0, .., n_{d-1}); unused leading axes have extent 1 and no stencil.
+ *    Wavefield rows are `pitch` elements apart (see nw_desc.pitch); the sound speed c,
+ *    initial pressure p0 and the outputs on the field grid are dense.
  *  - A wavefield buffer holds `n_shots` consecutive grids. PML auxiliary fields are face
  *    grids (extent + 1 along their own axis), also `n_shots` consecutive.
  *  - `dtype` selects float (0) or double (1) for every real-valued buffer of a plan.
@@ -30,7 +32,7 @@
 extern "C" {
 #endif
 
-#define NW_ABI_VERSION 1
+#define NW_ABI_VERSION 2
 
 enum nw_status {
   NW_OK = 0,
@@ -52,6 +54,12 @@ typedef struct nw_desc {
   int64_t dims[3];     /* padded grid extents (leading unused axes = 1) */
   int64_t n_abs[3];    /* absorbing cells per side along each axis (0 for unused axes) */
   int64_t field_dims[3]; /* physical (unpadded) grid = dims - 2*n_abs */
+  int64_t pitch;       /* elements between consecutive rows of axis 2 in every wavefield
+                        * buffer (>= dims[2]; 0 = dims[2]). A pitch that is a multiple of 32
+                        * keeps rows on 128-byte cache-line boundaries. Cell (z, y, x) of a
+                        * grid sits at (z*dims[1] + y)*pitch + x; columns x >= dims[2] are
+                        * never read or written. Flat cell indices below (src_idx, rec_idx,
+                        * inj_cell, nw_clamp's cells) use this pitched numbering. */
   double h[3];         /* grid spacing per axis (1.0 for unused axes) */
   double dt;           /* simulation time step */
   int32_t n_steps;     /* leapfrog steps of one simulation */
@@ -155,6 +163,14 @@ int32_t nw_plan_set_tiles(nw_plan* plan, int32_t kind, int32_t rows, int32_t sho
 int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
                           int32_t* shots_per_group);
 
+/* Plan options (no counterpart in the reference; autotuner knobs).
+ * NW_OPT_XCD_MAP: 1 (default) hands every XCD a contiguous run of the fast kernels' tiles so
+ * neighbouring row strips share an L2; 0 numbers workgroups linearly.
+ * NW_OPT_ROWS_PER_ITER: rows the fast forward step loads before computing (1 default, or 2). */
+enum nw_option { NW_OPT_XCD_MAP = 0, NW_OPT_ROWS_PER_ITER = 1 };
+int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
+int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
+
 /* Number of float64 gradient accumulator grids nw_adjoint writes (grad_m holds
  * nw_grad_slices * padded cells doubles; nw_model_grad sums all of them). */
 int64_t nw_grad_slices(const nw_plan* plan);
@@ -201,7 +217,8 @@ int32_t nw_ic_grad(nw_plan* plan, const void* m, const void* lam0, const void* p
 /* Adjoint of p^{-1} after the last reverse step: pbar = -b * a_eff * lam^1. */
 int32_t nw_prev_adjoint(nw_plan* plan, const void* lam1, void* pbar, void* stream);
 
-/* out = lap(p) over `count` grids of the plan's padded shape, zero outside (wave.py:144-168). */
+/* out = lap(p) over `count` grids of the plan's padded shape (pitched rows), zero outside
+ * (wave.py:144-168). */
 int32_t nw_laplacian(nw_plan* plan, const void* p, void* out, int64_t count, void* stream);
 
 /* Set p[s, cells[k]] = values[k] for k < n_cells in every shot: the receiver clamp of

@@ -22,7 +22,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libnefi_wave.so"
 NW_NONE, NW_SPONGE, NW_CERJAN, NW_PML = 0, 1, 2, 3
 NW_F32, NW_F64 = 0, 1
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # every symbol include/nefi_wave.h declares
 EXPORTS = (
@@ -47,6 +47,8 @@ EXPORTS = (
     "nw_plan_set_tiles",
     "nw_plan_get_tiles",
     "nw_grad_slices",
+    "nw_plan_set_option",
+    "nw_plan_get_option",
 )
 
 
@@ -61,6 +63,7 @@ class NwDesc(ctypes.Structure):
         ("dims", c_int64 * 3),
         ("n_abs", c_int64 * 3),
         ("field_dims", c_int64 * 3),
+        ("pitch", c_int64),
         ("h", c_double * 3),
         ("dt", c_double),
         ("n_steps", c_int32),
@@ -122,6 +125,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_plan_set_tiles": (i32, [vp, i32, i32, i32]),
         "nw_plan_get_tiles": (i32, [vp, i32, POINTER(c_int32), POINTER(c_int32)]),
         "nw_grad_slices": (i64, [vp]),
+        "nw_plan_set_option": (i32, [vp, i32, i32]),
+        "nw_plan_get_option": (i32, [vp, i32, POINTER(c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,6 +162,8 @@ def lib() -> ctypes.CDLL:
 
 
 K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT = 0, 1, 2, 3
+OPT_XCD_MAP = 0
+OPT_ROWS_PER_ITER = 1
 
 
 def profile_enable(on: bool) -> None:

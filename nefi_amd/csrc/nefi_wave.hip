@@ -86,7 +86,7 @@ static void prof_end(int id, hipStream_t st) {
 // ------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ int64_t stride_of(const Geo<T>& g, int a) {
-  return a == 0 ? g.D[1] * g.D[2] : (a == 1 ? g.D[2] : 1);
+  return a == 0 ? g.D[1] * g.P2 : (a == 1 ? g.P2 : 1);
 }
 
 // flat index of face f along axis a at the other coordinates of `co`
@@ -153,7 +153,7 @@ __device__ __forceinline__ bool cell_of(const Geo<T>& g, int64_t& s, int64_t* co
   s = zs / g.D[0];
   co[1] = y;
   co[2] = x;
-  idx = (co[0] * g.D[1] + y) * g.D[2] + x;
+  idx = (co[0] * g.D[1] + y) * g.P2 + x;
   return true;
 }
 
@@ -356,14 +356,14 @@ __global__ void __launch_bounds__(256) k_model_grad(Geo<T> g, const T* __restric
     const double* G = gm + s * g.cells;
     for (int64_t z = lo0; z <= hi0; ++z)
       for (int64_t y = lo1; y <= hi1; ++y)
-        for (int64_t x = lo2; x <= hi2; ++x) acc += G[(z * g.D[1] + y) * g.D[2] + x];
+        for (int64_t x = lo2; x <= hi2; ++x) acc += G[(z * g.D[1] + y) * g.P2 + x];
   }
   gc[t] = (T)(2.0 * (double)g.dt2_d * (double)c[t] * acc);
 }
 
 template <typename T>
 __device__ __forceinline__ T p0_at(const Geo<T>& g, const T* p0s, int64_t j) {
-  const int64_t x = j % g.D[2], y = (j / g.D[2]) % g.D[1], z = j / (g.D[1] * g.D[2]);
+  const int64_t x = j % g.P2, y = (j / g.P2) % g.D[1], z = j / (g.D[1] * g.P2);
   const int64_t i0 = z - g.nabs[0], i1 = y - g.nabs[1], i2 = x - g.nabs[2];
   if (i0 < 0 || i1 < 0 || i2 < 0 || i0 >= g.F[0] || i1 >= g.F[1] || i2 >= g.F[2]) return T(0);
   return p0s[(i0 * g.F[1] + i1) * g.F[2] + i2];
@@ -435,7 +435,8 @@ Geo<T> make_geo(const nw_desc& d) {
     g.nabs[a] = d.n_abs[a];
     g.used[a] = a >= 3 - d.ndim;
   }
-  g.cells = g.D[0] * g.D[1] * g.D[2];
+  g.P2 = d.pitch > 0 ? d.pitch : d.dims[2];
+  g.cells = g.D[0] * g.D[1] * g.P2;
   for (int a = 0; a < 3; ++a) {
     int64_t f = 1;
     for (int b = 0; b < 3; ++b) f *= g.D[b] + (b == a);
@@ -479,6 +480,8 @@ struct nw_plan {
   // advanced by one workgroup; [0] forward step, [1] reverse step
   int rows[2];
   int spw[2];
+  int xcd_map;  // NW_OPT_XCD_MAP
+  int rows_per_iter;  // NW_OPT_ROWS_PER_ITER (forward step)
   int64_t shot_bytes;          // one wavefield (all shots)
   int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
 };
@@ -588,7 +591,8 @@ template <typename T>
 static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
   if constexpr (std::is_same<T, float>::value) {
     if (P->rows[0] > 0) {
-      fast2d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[0], P->spw[0], st);
+      fast2d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[0], P->spw[0],
+                  P->xcd_map | (P->rows_per_iter == 2 ? 2 : 0), st);
       return;
     }
   }
@@ -610,7 +614,7 @@ static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
   if constexpr (std::is_same<T, float>::value) {
     if (P->rows[1] > 0) {
       fast2d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[1], P->spw[1],
-                     st);
+                     P->xcd_map, st);
       return;
     }
   }
@@ -901,7 +905,8 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
     if (d.field_dims[a] + 2 * d.n_abs[a] != d.dims[a])
       return fail(NW_ERR_ARG, "dims must equal field_dims + 2*n_abs");
   }
-  const int64_t cells = d.dims[0] * d.dims[1] * d.dims[2];
+  if (d.pitch != 0 && d.pitch < d.dims[2]) return fail(NW_ERR_ARG, "pitch must be >= dims[2]");
+  const int64_t cells = d.dims[0] * d.dims[1] * (d.pitch > 0 ? d.pitch : d.dims[2]);
   if (cells >= (int64_t)1 << 31) return fail(NW_ERR_ARG, "padded grid exceeds 2^31 cells");
   if (d.dims[0] * d.n_shots > 65535) return fail(NW_ERR_ARG, "dims[0] * n_shots exceeds 65535");
   if (d.rec_max > 0 && (d.rec_taps != (1 << d.ndim) || !d.rec_idx || !d.rec_w))
@@ -925,6 +930,8 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
     p->rows[k] = fast ? 16 : 0;
     p->spw[k] = 1;
   }
+  p->xcd_map = 1;
+  p->rows_per_iter = 1;
   const size_t es = d.dtype == NW_F64 ? 8 : 4;
   p->shot_bytes = cells * (int64_t)es * d.n_shots;
   for (int a = 0; a < 3; ++a)
@@ -969,6 +976,33 @@ int32_t nw_plan_get_tiles(const nw_plan* P, int32_t kind, int32_t* rows, int32_t
   *rows = P->rows[kind];
   *shots_per_group = P->spw[kind];
   return NW_OK;
+}
+
+int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
+  if (!P) return fail(NW_ERR_ARG, "nw_plan_set_option: null plan");
+  if (option == NW_OPT_XCD_MAP) {
+    P->xcd_map = value != 0;
+    return NW_OK;
+  }
+  if (option == NW_OPT_ROWS_PER_ITER) {
+    if (value != 1 && value != 2) return fail(NW_ERR_ARG, "rows per iteration must be 1 or 2");
+    P->rows_per_iter = value;
+    return NW_OK;
+  }
+  return fail(NW_ERR_ARG, "nw_plan_set_option: unknown option");
+}
+
+int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
+  if (!P || !value) return fail(NW_ERR_ARG, "nw_plan_get_option: null argument");
+  if (option == NW_OPT_XCD_MAP) {
+    *value = P->xcd_map;
+    return NW_OK;
+  }
+  if (option == NW_OPT_ROWS_PER_ITER) {
+    *value = P->rows_per_iter;
+    return NW_OK;
+  }
+  return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");
 }
 
 int64_t nw_ckpt_slot_bytes(const nw_plan* P) {
@@ -1143,7 +1177,7 @@ int32_t nw_clamp(nw_plan* P, void* p, const int32_t* cells, const void* values, 
   if (!P || !p || !cells || !values || n_cells < 0) return fail(NW_ERR_ARG, "nw_clamp: bad argument");
   if (n_cells == 0) return NW_OK;
   hipStream_t st = (hipStream_t)stream;
-  const int64_t cellsz = P->d.dims[0] * P->d.dims[1] * P->d.dims[2];
+  const int64_t cellsz = P->gf.cells;
   const dim3 grd((n_cells + 255) / 256, P->d.n_shots);
   if (P->d.dtype == NW_F32)
     hipLaunchKernelGGL(k_clamp<float>, grd, dim3(256), 0, st, cellsz, (float*)p, cells, (const float*)values, n_cells);

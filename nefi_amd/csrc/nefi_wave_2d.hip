@@ -34,48 +34,10 @@ __device__ __forceinline__ float comp(const float4& v, int k) {
 }
 
 // row y of a grid (zero outside the grid or for inactive lanes)
-__device__ __forceinline__ float4 row4(const float* g, int y, int D1, int D2, int x, bool act) {
-  return (act && y >= 0 && y < D1) ? ld4(g + (int64_t)y * D2 + x) : f4(0.f);
-}
-
-// x-neighbours of a row vector v held by every lane at columns x..x+3: returns the values at
-// x-2, x-1 (l2, l1) and x+4, x+5 (r1, r2). Lanes 0 and 63 read their outer columns from `g`.
-__device__ __forceinline__ void xhalo(const float4& v, const float* g, int y, int D2, int x,
-                                      int lane, float& l2, float& l1, float& r1, float& r2) {
-  l2 = __shfl_up(v.z, 1, 64);
-  l1 = __shfl_up(v.w, 1, 64);
-  r1 = __shfl_down(v.x, 1, 64);
-  r2 = __shfl_down(v.y, 1, 64);
-  if (lane == 0) {
-    const float* rp = g + (int64_t)y * D2;
-    l2 = x - 2 >= 0 ? rp[x - 2] : 0.f;
-    l1 = x - 1 >= 0 ? rp[x - 1] : 0.f;
-  }
-  if (lane == 63) {
-    const float* rp = g + (int64_t)y * D2;
-    r1 = x + 4 < D2 ? rp[x + 4] : 0.f;
-    r2 = x + 5 < D2 ? rp[x + 5] : 0.f;
-  }
-}
-
-// same for a product of two grids (q = a*b), used by the reverse sweep for m*a_eff*lambda
-__device__ __forceinline__ void xhalo_prod(const float4& v, const float* ga, const float* gb,
-                                           int y, int D2, int x, int lane, float& l2, float& l1,
-                                           float& r1, float& r2) {
-  l2 = __shfl_up(v.z, 1, 64);
-  l1 = __shfl_up(v.w, 1, 64);
-  r1 = __shfl_down(v.x, 1, 64);
-  r2 = __shfl_down(v.y, 1, 64);
-  if (lane == 0) {
-    const int64_t o = (int64_t)y * D2;
-    l2 = x - 2 >= 0 ? ga[o + x - 2] * gb[o + x - 2] : 0.f;
-    l1 = x - 1 >= 0 ? ga[o + x - 1] * gb[o + x - 1] : 0.f;
-  }
-  if (lane == 63) {
-    const int64_t o = (int64_t)y * D2;
-    r1 = x + 4 < D2 ? ga[o + x + 4] * gb[o + x + 4] : 0.f;
-    r2 = x + 5 < D2 ? ga[o + x + 5] * gb[o + x + 5] : 0.f;
-  }
+// offsets within one shot's grid fit in 32 bits (nw_plan_create rejects >= 2^31 cells), which
+// lets loads use a uniform base address plus a 32-bit per-lane offset
+__device__ __forceinline__ float4 row4(const float* g, int y, int D1, int P2, int x, bool act) {
+  return (act && y >= 0 && y < D1) ? ld4(g + (y * P2 + x)) : f4(0.f);
 }
 
 // one component of the cross stencil, in the generic kernels' order (axis 1, then axis 2)
@@ -128,77 +90,195 @@ __device__ __forceinline__ void damp1(float hy, float hx, float& a, float& b, fl
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
+// tile mapping
+// ------------------------------------------------------------------------------------------
+// MI355X places workgroup i of a dispatch on XCD i % 8 and every XCD has its own L2. Tiles are
+// numbered column chunk fastest, then row strip, then shot group, and XCD j is given the
+// contiguous range [j*per, (j+1)*per) of that order, so the strips above and below a tile
+// (whose rows form its halo) run on the same XCD and share its L2.
+constexpr int kXcds = 8;
+
+struct Tile {
+  int cx, ry, grp;
+  bool valid;
+};
+
+__device__ __forceinline__ Tile tile_of(int nX, int nY, int nG, int xcd_map) {
+  const int T = nX * nY * nG;
+  const int per = (T + kXcds - 1) / kXcds;
+  const int i = blockIdx.x;
+  const int t = xcd_map ? (i % kXcds) * per + i / kXcds : i;
+  Tile r;
+  r.valid = t < T;
+  r.cx = t % nX;
+  r.ry = (t / nX) % nY;
+  r.grp = t / (nX * nY);
+  return r;
+}
+
+__host__ __device__ inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ float2 ld2(const float* p) {
+  return *reinterpret_cast<const float2*>(p);
+}
+
+// the two outer columns lanes 0 and 63 need (x-2, x-1 for lane 0; x+4, x+5 for lane 63),
+// as one 8-byte load per row; zero outside the grid and for the other lanes
+__device__ __forceinline__ float2 edge2(const float* g, int y, int D2, int P2, int x, int lane) {
+  const float* rp = g + y * P2;
+  if (lane == 0 && x >= 4) return ld2(rp + x - 2);
+  if (lane == 63 && x + 4 < D2) return ld2(rp + x + 4);
+  return make_float2(0.f, 0.f);
+}
+
+// x-neighbours of the lane's four columns: (x-2, x-1, x+4, x+5) from the neighbouring lanes,
+// with lanes 0 and 63 taking them from their edge pair. Returned by value and chosen with
+// selects so the compiler keeps them in registers.
+__device__ __forceinline__ float4 xnb(const float4& v, const float2& e, int lane) {
+  const float l2 = __shfl_up(v.z, 1, 64);
+  const float l1 = __shfl_up(v.w, 1, 64);
+  const float r1 = __shfl_down(v.x, 1, 64);
+  const float r2 = __shfl_down(v.y, 1, 64);
+  return make_float4(lane == 0 ? e.x : l2, lane == 0 ? e.y : l1, lane == 63 ? e.x : r1,
+                     lane == 63 ? e.y : r2);
+}
+
+// ------------------------------------------------------------------------------------------
+// per-cell arithmetic on the four columns of a lane (plain float4 values: nothing indexable,
+// so nothing is placed in scratch memory)
+// ------------------------------------------------------------------------------------------
+template <int ABS>
+struct Damp4 {
+  float4 a, b, tap;
+};
+
+template <int ABS>
+__device__ __forceinline__ Damp4<ABS> damp4(float hy, const float4& hx) {
+  Damp4<ABS> d;
+  damp1<ABS>(hy, hx.x, d.a.x, d.b.x, d.tap.x);
+  damp1<ABS>(hy, hx.y, d.a.y, d.b.y, d.tap.y);
+  damp1<ABS>(hy, hx.z, d.a.z, d.b.z, d.tap.z);
+  damp1<ABS>(hy, hx.w, d.a.w, d.b.w, d.tap.w);
+  return d;
+}
+
+// rhs = (2 p - b p_prev~) + m lap, with p_prev~ = tap * p_prev for a tapered previous state
+template <int ABS>
+__device__ __forceinline__ float rhs1(float pc, float prev_raw, float m, float lap, float b,
+                                      float tap, int taper_prev) {
+  const float prev = (ABS == NW_CERJAN && taper_prev) ? tap * prev_raw : prev_raw;
+  return (2.f * pc - b * prev) + m * lap;
+}
+
+// adjoint of one cell: lambda^n from rbar = a lambda^{n+1}, the next step's lambda^{n+2} term
+// and the stencil of q
+template <int ABS>
+__device__ __forceinline__ float adj1(float lc, float l2c, float lapq, float a, float b,
+                                      float tap, int taper_next, float& rbar) {
+  rbar = a * lc;
+  const float tp1 = (ABS == NW_CERJAN && taper_next) ? tap : 1.f;
+  const float pbar_next = -(b * tp1) * (a * l2c);
+  return pbar_next + 2.f * rbar + lapq;
+}
+
+__device__ __forceinline__ float4 mul4(const float4& a, const float4& b) {
+  return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
+}
+
+// ------------------------------------------------------------------------------------------
 // forward step
 // ------------------------------------------------------------------------------------------
 template <int R, int ABS, int TY>
+__device__ __forceinline__ void step_row(const Geo<float>& g, const StepIO<float>& io, int s,
+                                         int64_t base, int y, int x, int P2, int lane, bool act,
+                                         const float4& hx, float w_n, float4* w, const float4& pp,
+                                         const float4& mm, const float2& ed) {
+  const float4 nb = xnb(w[R], ed, lane);
+  if (!act) return;
+  const int ro = y * P2 + x;
+  const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+  const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+  const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+  const float4 pc = w[R];
+  const int tpv = io.taper_prev;
+  float4 o = make_float4(rhs1<ABS>(pc.x, pp.x, mm.x, lap.x, dm.b.x, dm.tap.x, tpv),
+                         rhs1<ABS>(pc.y, pp.y, mm.y, lap.y, dm.b.y, dm.tap.y, tpv),
+                         rhs1<ABS>(pc.z, pp.z, mm.z, lap.z, dm.b.z, dm.tap.z, tpv),
+                         rhs1<ABS>(pc.w, pp.w, mm.w, lap.w, dm.b.w, dm.tap.w, tpv));
+  for (int t = 0; t < io.src_taps; ++t) {
+    const int d = io.src_idx[s * io.src_taps + t] - ro;
+    if (d >= 0 && d < 4) {
+      const float v = io.src_val[s * io.src_taps + t] * w_n;
+      o.x += d == 0 ? v : 0.f;
+      o.y += d == 1 ? v : 0.f;
+      o.z += d == 2 ? v : 0.f;
+      o.w += d == 3 ? v : 0.f;
+    }
+  }
+  if (ABS != NW_NONE) o = mul4(dm.a, o);
+  st4(io.p_next + base + ro, o);
+  if (io.ck_prev) {
+    st4(io.ck_prev + base + ro, pp);
+    st4(io.ck_cur + base + ro, pc);
+  }
+}
+
+template <int R, int ABS, int TY, int U>
 __global__ void __launch_bounds__(256) k2_step(Geo<float> g, StepIO<float> io, int shots,
-                                               int spw) {
+                                               int spw, int xcd_map) {
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, 4 * TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
-  const int x = (blockIdx.x * 64 + lane) * 4;
-  const int ys = (blockIdx.y * 4 + wave) * TY;
+  const int x = (tl.cx * 64 + lane) * 4;
+  const int ys = (tl.ry * 4 + wave) * TY;
   if (ys >= D1) return;
   const int ye = ys + TY < D1 ? ys + TY : D1;
   const bool act = x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
-  const int s_end = min(shots, (int)(blockIdx.z + 1) * spw);
-  for (int s = blockIdx.z * spw; s < s_end; ++s) {
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
     const int64_t base = (int64_t)s * g.cells;
     const float* P = io.p + base;
+    const float* PP = io.p_prev + base;
     float4 w[2 * R + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * R; ++k) w[k] = row4(P, ys - R + k, D1, D2, x, act);
-#pragma unroll 2
-    for (int y = ys; y < ye; ++y) {
-      w[2 * R] = row4(P, y + R, D1, D2, x, act);
-      const int64_t ro = (int64_t)y * D2 + x;
-      float l2, l1, r1, r2;
-      xhalo(w[R], P, y, D2, x, lane, l2, l1, r1, r2);
-      if (act) {
-        const float4 lap = stencil4<R>(g, w, l2, l1, r1, r2);
-        const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const float4 pp = ld4(io.p_prev + base + ro);
-        const float4 mm = ld4(io.m + ro);
-        const float4 pc = w[R];
-        float4 out;
-        float o[4];
+    for (int k = 0; k < 2 * R; ++k) w[k] = row4(P, ys - R + k, D1, P2, x, act);
+    if (U == 1) {
+      for (int y = ys; y < ye; ++y) {
+        w[2 * R] = row4(P, y + R, D1, P2, x, act);
+        const float4 pp = act ? ld4(PP + (y * P2 + x)) : f4(0.f);
+        const float4 mm = act ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+        const float2 ed = edge2(P, y, D2, P2, x, lane);
+        step_row<R, ABS, TY>(g, io, s, base, y, x, P2, lane, act, hx, w_n, w, pp, mm, ed);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float a, b, tap;
-          damp1<ABS>(hy, comp(hx, k), a, b, tap);
-          const float prev_raw = comp(pp, k);
-          const float prev = (ABS == NW_CERJAN && io.taper_prev) ? tap * prev_raw : prev_raw;
-          const float rhs = (2.f * comp(pc, k) - b * prev) + comp(mm, k) * comp(lap, k);
-          o[k] = rhs;
-        }
-        if (io.src_taps > 0) {
-          for (int t = 0; t < io.src_taps; ++t) {
-            const int64_t d = (int64_t)io.src_idx[s * io.src_taps + t] - ro;
-            if (d >= 0 && d < 4) {
-              const float v = io.src_val[s * io.src_taps + t] * w_n;
-              if (d == 0) o[0] += v;
-              else if (d == 1) o[1] += v;
-              else if (d == 2) o[2] += v;
-              else o[3] += v;
-            }
-          }
-        }
+        for (int k = 0; k < 2 * R; ++k) w[k] = w[k + 1];
+      }
+    } else {
+      for (int y = ys; y < ye; y += 2) {
+        const bool two = y + 1 < ye;  // uniform across the wavefront
+        // every load of both rows before any use
+        const float4 n0 = row4(P, y + R, D1, P2, x, act);
+        const float4 n1 = two ? row4(P, y + R + 1, D1, P2, x, act) : f4(0.f);
+        const float4 pp0 = act ? ld4(PP + (y * P2 + x)) : f4(0.f);
+        const float4 pp1 = (act && two) ? ld4(PP + ((y + 1) * P2 + x)) : f4(0.f);
+        const float4 m0 = act ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+        const float4 m1 = (act && two) ? ld4(io.m + ((y + 1) * P2 + x)) : f4(0.f);
+        const float2 e0 = edge2(P, y, D2, P2, x, lane);
+        const float2 e1 = two ? edge2(P, y + 1, D2, P2, x, lane) : make_float2(0.f, 0.f);
+        w[2 * R] = n0;
+        step_row<R, ABS, TY>(g, io, s, base, y, x, P2, lane, act, hx, w_n, w, pp0, m0, e0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float a, b, tap;
-          damp1<ABS>(hy, comp(hx, k), a, b, tap);
-          if (ABS != NW_NONE) o[k] = a * o[k];
-        }
-        out = make_float4(o[0], o[1], o[2], o[3]);
-        st4(io.p_next + base + ro, out);
-        if (io.ck_prev) {
-          st4(io.ck_prev + base + ro, pp);
-          st4(io.ck_cur + base + ro, pc);
+        for (int k = 0; k < 2 * R; ++k) w[k] = w[k + 1];
+        if (two) {
+          w[2 * R] = n1;
+          step_row<R, ABS, TY>(g, io, s, base, y + 1, x, P2, lane, act, hx, w_n, w, pp1, m1,
+                               e1);
+#pragma unroll
+          for (int k = 0; k < 2 * R; ++k) w[k] = w[k + 1];
         }
       }
-#pragma unroll
-      for (int k = 0; k < 2 * R; ++k) w[k] = w[k + 1];
     }
   }
 }
@@ -208,78 +288,78 @@ __global__ void __launch_bounds__(256) k2_step(Geo<float> g, StepIO<float> io, i
 // ------------------------------------------------------------------------------------------
 template <int R, int ABS, int TY>
 __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io, int shots,
-                                                  int spw) {
+                                                  int spw, int xcd_map) {
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, 4 * TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
-  const int x = (blockIdx.x * 64 + lane) * 4;
-  const int ys = (blockIdx.y * 4 + wave) * TY;
+  const int x = (tl.cx * 64 + lane) * 4;
+  const int ys = (tl.ry * 4 + wave) * TY;
   if (ys >= D1) return;
   const int ye = ys + TY < D1 ? ys + TY : D1;
   const bool act = x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
-  const int s_end = min(shots, (int)(blockIdx.z + 1) * spw);
+  const int s_end = min(shots, (tl.grp + 1) * spw);
   // gradient slice of this shot group (one float64 accumulator grid per group)
-  double* G = io.grad ? io.grad + (int64_t)blockIdx.z * g.cells : nullptr;
-  for (int s = blockIdx.z * spw; s < s_end; ++s) {
+  double* G = io.grad ? io.grad + (int64_t)tl.grp * g.cells : nullptr;
+  const float* MA = io.ma;
+  const int tpn = io.taper_next;
+  for (int s = tl.grp * spw; s < s_end; ++s) {
     const int64_t base = (int64_t)s * g.cells;
     const float* L1 = io.lam1 + base;
-    const float* MA = io.ma;
     const float* Pn = io.p_n ? io.p_n + base : nullptr;
-    float4 lw[2 * R + 1], mw[2 * R + 1], pw[2 * R + 1];
+    // window of q = (m a_eff) * lambda^{n+1}, formed as the rows arrive
+    float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
-      lw[k] = row4(L1, ys - R + k, D1, D2, x, act);
-      mw[k] = row4(MA, ys - R + k, D1, D2, x, act);
-      pw[k] = Pn ? row4(Pn, ys - R + k, D1, D2, x, act) : f4(0.f);
+      qw[k] = mul4(row4(MA, ys - R + k, D1, P2, x, act), row4(L1, ys - R + k, D1, P2, x, act));
+      pw[k] = Pn ? row4(Pn, ys - R + k, D1, P2, x, act) : f4(0.f);
     }
-#pragma unroll 2
     for (int y = ys; y < ye; ++y) {
-      lw[2 * R] = row4(L1, y + R, D1, D2, x, act);
-      mw[2 * R] = row4(MA, y + R, D1, D2, x, act);
-      if (Pn) pw[2 * R] = row4(Pn, y + R, D1, D2, x, act);
-      const int64_t ro = (int64_t)y * D2 + x;
-      float4 qw[2 * R + 1];
-#pragma unroll
-      for (int k = 0; k < 2 * R + 1; ++k)
-        qw[k] = make_float4(mw[k].x * lw[k].x, mw[k].y * lw[k].y, mw[k].z * lw[k].z,
-                            mw[k].w * lw[k].w);
-      float l2, l1, r1, r2;
-      xhalo_prod(qw[R], MA, L1, y, D2, x, lane, l2, l1, r1, r2);
-      float pl2 = 0.f, pl1 = 0.f, pr1 = 0.f, pr2 = 0.f;
-      if (Pn) xhalo(pw[R], Pn, y, D2, x, lane, pl2, pl1, pr1, pr2);
+      const int ro = y * P2 + x;
+      // every load of the row first
+      const float4 ln = row4(L1, y + R, D1, P2, x, act);
+      const float4 mn = row4(MA, y + R, D1, P2, x, act);
+      const float4 pn = Pn ? row4(Pn, y + R, D1, P2, x, act) : f4(0.f);
+      const float4 l2c = act ? ld4(io.lam2 + base + ro) : f4(0.f);
+      const float4 lc = act ? ld4(L1 + ro) : f4(0.f);  // read R rows ago: a cache hit
+      const float2 el = edge2(L1, y, D2, P2, x, lane);
+      const float2 em = edge2(MA, y, D2, P2, x, lane);
+      const float2 ep = Pn ? edge2(Pn, y, D2, P2, x, lane) : make_float2(0.f, 0.f);
+      double2* gp = (G && Pn && act) ? reinterpret_cast<double2*>(G + ro) : nullptr;
+      double2 g0 = make_double2(0.0, 0.0), g1 = g0;
+      if (gp) {
+        g0 = gp[0];
+        g1 = gp[1];
+      }
+      qw[2 * R] = mul4(mn, ln);
+      pw[2 * R] = pn;
+      const float4 qn = xnb(qw[R], make_float2(em.x * el.x, em.y * el.y), lane);
+      const float4 pnb = Pn ? xnb(pw[R], ep, lane) : f4(0.f);
       if (act) {
-        const float4 lapq = stencil4<R>(g, qw, l2, l1, r1, r2);
+        const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
         const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const float4 l2c = ld4(io.lam2 + base + ro);
-        const float4 lc = lw[R];
-        float o[4], rb[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float a, b, tap;
-          damp1<ABS>(hy, comp(hx, k), a, b, tap);
-          const float rbar = a * comp(lc, k);
-          const float tp1 = (ABS == NW_CERJAN && io.taper_next) ? tap : 1.f;
-          const float pbar_next = -(b * tp1) * (a * comp(l2c, k));
-          o[k] = pbar_next + 2.f * rbar + comp(lapq, k);
-          rb[k] = rbar;
-        }
-        st4(io.lam0 + base + ro, make_float4(o[0], o[1], o[2], o[3]));
-        if (Pn && G) {
-          const float4 lapp = stencil4<R>(g, pw, pl2, pl1, pr1, pr2);
-          double2* gp = reinterpret_cast<double2*>(G + ro);
-          double2 g0 = gp[0], g1 = gp[1];
-          g0.x += (double)rb[0] * (double)lapp.x;
-          g0.y += (double)rb[1] * (double)lapp.y;
-          g1.x += (double)rb[2] * (double)lapp.z;
-          g1.y += (double)rb[3] * (double)lapp.w;
+        const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+        float4 rb;
+        const float4 o = make_float4(
+            adj1<ABS>(lc.x, l2c.x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
+            adj1<ABS>(lc.y, l2c.y, lapq.y, dm.a.y, dm.b.y, dm.tap.y, tpn, rb.y),
+            adj1<ABS>(lc.z, l2c.z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
+            adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
+        st4(io.lam0 + base + ro, o);
+        if (gp) {
+          const float4 lapp = stencil4<R>(g, pw, pnb.x, pnb.y, pnb.z, pnb.w);
+          g0.x += (double)rb.x * (double)lapp.x;
+          g0.y += (double)rb.y * (double)lapp.y;
+          g1.x += (double)rb.z * (double)lapp.z;
+          g1.y += (double)rb.w * (double)lapp.w;
           gp[0] = g0;
           gp[1] = g1;
         }
       }
 #pragma unroll
       for (int k = 0; k < 2 * R; ++k) {
-        lw[k] = lw[k + 1];
-        mw[k] = mw[k + 1];
+        qw[k] = qw[k + 1];
         pw[k] = pw[k + 1];
       }
     }
@@ -293,71 +373,80 @@ bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype) {
   return g.nd == 2 && dtype == NW_F32 && absorbing != NW_PML && g.D[2] % 4 == 0;
 }
 
-template <int R, int ABS, int TY>
-static void launch_step_t(const Geo<float>& g, const StepIO<float>& io, int shots, int spw,
-                          hipStream_t st) {
-  const dim3 grid((unsigned)((g.D[2] + 255) / 256), (unsigned)((g.D[1] + 4 * TY - 1) / (4 * TY)),
-                  (unsigned)((shots + spw - 1) / spw));
-  hipLaunchKernelGGL((k2_step<R, ABS, TY>), grid, dim3(256), 0, st, g, io, shots, spw);
+// 1-D grid of kXcds * ceil(tiles / kXcds) workgroups (tile_of maps them, a few return)
+static unsigned grid_tiles(const Geo<float>& g, int ty, int shots, int spw) {
+  const int T = div_up((int)g.D[2], 256) * div_up((int)g.D[1], 4 * ty) * div_up(shots, spw);
+  return (unsigned)(kXcds * div_up(T, kXcds));
 }
 
 template <int R, int ABS, int TY>
-static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int spw,
+static void launch_step_t(const Geo<float>& g, const StepIO<float>& io, int shots, int spw,
+                          int xm, hipStream_t st) {
+  // xm bit 1 selects the two-rows-per-iteration march
+  if (xm & 2)
+    hipLaunchKernelGGL((k2_step<R, ABS, TY, 2>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
+                       0, st, g, io, shots, spw, xm & 1);
+  else
+    hipLaunchKernelGGL((k2_step<R, ABS, TY, 1>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
+                       0, st, g, io, shots, spw, xm & 1);
+}
+
+template <int R, int ABS, int TY>
+static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int spw, int xm,
                          hipStream_t st) {
-  const dim3 grid((unsigned)((g.D[2] + 255) / 256), (unsigned)((g.D[1] + 4 * TY - 1) / (4 * TY)),
-                  (unsigned)((shots + spw - 1) / spw));
-  hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), grid, dim3(256), 0, st, g, io, shots, spw);
+  hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256), 0,
+                     st, g, io, shots, spw, xm & 1);
 }
 
 template <int R, int ABS>
 static void step_ty(const Geo<float>& g, const StepIO<float>& io, int shots, int ty, int spw,
-                    hipStream_t st) {
+                    int xm, hipStream_t st) {
   switch (ty) {
-    case 4: launch_step_t<R, ABS, 4>(g, io, shots, spw, st); break;
-    case 8: launch_step_t<R, ABS, 8>(g, io, shots, spw, st); break;
-    case 32: launch_step_t<R, ABS, 32>(g, io, shots, spw, st); break;
-    case 64: launch_step_t<R, ABS, 64>(g, io, shots, spw, st); break;
-    default: launch_step_t<R, ABS, 16>(g, io, shots, spw, st); break;
+    case 4: launch_step_t<R, ABS, 4>(g, io, shots, spw, xm, st); break;
+    case 8: launch_step_t<R, ABS, 8>(g, io, shots, spw, xm, st); break;
+    case 32: launch_step_t<R, ABS, 32>(g, io, shots, spw, xm, st); break;
+    case 64: launch_step_t<R, ABS, 64>(g, io, shots, spw, xm, st); break;
+    default: launch_step_t<R, ABS, 16>(g, io, shots, spw, xm, st); break;
   }
 }
 
 template <int R, int ABS>
 static void adj_ty(const Geo<float>& g, const AdjIO<float>& io, int shots, int ty, int spw,
-                   hipStream_t st) {
+                   int xm, hipStream_t st) {
   switch (ty) {
-    case 4: launch_adj_t<R, ABS, 4>(g, io, shots, spw, st); break;
-    case 8: launch_adj_t<R, ABS, 8>(g, io, shots, spw, st); break;
-    case 32: launch_adj_t<R, ABS, 32>(g, io, shots, spw, st); break;
-    case 64: launch_adj_t<R, ABS, 64>(g, io, shots, spw, st); break;
-    default: launch_adj_t<R, ABS, 16>(g, io, shots, spw, st); break;
+    case 4: launch_adj_t<R, ABS, 4>(g, io, shots, spw, xm, st); break;
+    case 8: launch_adj_t<R, ABS, 8>(g, io, shots, spw, xm, st); break;
+    case 32: launch_adj_t<R, ABS, 32>(g, io, shots, spw, xm, st); break;
+    case 64: launch_adj_t<R, ABS, 64>(g, io, shots, spw, xm, st); break;
+    default: launch_adj_t<R, ABS, 16>(g, io, shots, spw, xm, st); break;
   }
 }
 
 void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io,
-                 int shots, int ty, int spw, void* stream) {
+                 int shots, int ty, int spw, int xm, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const int key = absorbing * 2 + (order == 4 ? 1 : 0);
   switch (key) {
-    case NW_NONE * 2 + 0: step_ty<1, NW_NONE>(g, io, shots, ty, spw, st); break;
-    case NW_NONE * 2 + 1: step_ty<2, NW_NONE>(g, io, shots, ty, spw, st); break;
-    case NW_SPONGE * 2 + 0: step_ty<1, NW_SPONGE>(g, io, shots, ty, spw, st); break;
-    case NW_SPONGE * 2 + 1: step_ty<2, NW_SPONGE>(g, io, shots, ty, spw, st); break;
-    case NW_CERJAN * 2 + 0: step_ty<1, NW_CERJAN>(g, io, shots, ty, spw, st); break;
-    default: step_ty<2, NW_CERJAN>(g, io, shots, ty, spw, st); break;
+    case NW_NONE * 2 + 0: step_ty<1, NW_NONE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_NONE * 2 + 1: step_ty<2, NW_NONE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_SPONGE * 2 + 0: step_ty<1, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_SPONGE * 2 + 1: step_ty<2, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_CERJAN * 2 + 0: step_ty<1, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    default: step_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
   }
 }
 
 void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
-                    int shots, int ty, int spw, void* stream) {
+                    int shots, int ty, int spw, int xm, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const int key = absorbing * 2 + (order == 4 ? 1 : 0);
   switch (key) {
-    case NW_NONE * 2 + 0: adj_ty<1, NW_NONE>(g, io, shots, ty, spw, st); break;
-    case NW_NONE * 2 + 1: adj_ty<2, NW_NONE>(g, io, shots, ty, spw, st); break;
-    case NW_SPONGE * 2 + 0: adj_ty<1, NW_SPONGE>(g, io, shots, ty, spw, st); break;
-    case NW_SPONGE * 2 + 1: adj_ty<2, NW_SPONGE>(g, io, shots, ty, spw, st); break;
-    case NW_CERJAN * 2 + 0: adj_ty<1, NW_CERJAN>(g, io, shots, ty, spw, st); break;
-    default: adj_ty<2, NW_CERJAN>(g, io, shots, ty, spw, st); break;
+    case NW_NONE * 2 + 0: adj_ty<1, NW_NONE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_NONE * 2 + 1: adj_ty<2, NW_NONE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_SPONGE * 2 + 0: adj_ty<1, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_SPONGE * 2 + 1: adj_ty<2, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
+    case NW_CERJAN * 2 + 0: adj_ty<1, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    default: adj_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
   }
 }
 

@@ -15,7 +15,8 @@ struct Geo {
   int64_t D[3];       // padded extents, axis 2 fastest
   int64_t F[3];       // field (unpadded) extents
   int64_t nabs[3];    // absorbing cells per side
-  int64_t cells;      // D0*D1*D2
+  int64_t P2;         // row pitch of axis 2 (>= D[2])
+  int64_t cells;      // D0*D1*P2: elements of one shot's wavefield
   int64_t fcells[3];  // cells of the face grid of each axis
   int used[3];        // axis carries a stencil
   int nd;             // physical dimensions
@@ -94,8 +95,8 @@ struct AdjIO {
 // fast 2-D float32 kernels (nefi_wave_2d.hip)
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype);
 void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
-                 int ty, int spw, void* stream);
+                 int ty, int spw, int xcd_map, void* stream);
 void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
-                    int shots, int ty, int spw, void* stream);
+                    int shots, int ty, int spw, int xcd_map, void* stream);
 
 }  // namespace nw

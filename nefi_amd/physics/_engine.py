@@ -53,10 +53,29 @@ class HostPlan:
     rec_idx: torch.Tensor | None = None  # (rows, taps) int64
     rec_w: torch.Tensor | None = None  # (rows, taps) f64
     n_trace_rows: int = 0
+    pitch: int = 0  # row pitch of wavefield buffers (0 = dims[2])
+
+    @property
+    def row_pitch(self) -> int:
+        return self.pitch or self.dims[2]
 
     @property
     def cells(self) -> int:
-        return self.dims[0] * self.dims[1] * self.dims[2]
+        """Elements of one shot's (pitched) wavefield."""
+        return self.dims[0] * self.dims[1] * self.row_pitch
+
+    def pitched(self, idx: torch.Tensor) -> torch.Tensor:
+        """Dense flat cell indices of the padded grid -> indices in pitched buffers."""
+        idx = torch.as_tensor(idx, dtype=torch.int64)
+        d2 = self.dims[2]
+        out = (idx // d2) * self.row_pitch + idx % d2
+        return torch.where(idx < 0, idx, out)
+
+    def wavefield_view(self, flat: torch.Tensor, batch: int) -> torch.Tensor:
+        """``(batch, *padded)`` view of a pitched flat buffer (padding columns dropped)."""
+        lead = self.dims[3 - self.ndim : 2]
+        v = flat.view(batch, *lead, self.row_pitch)
+        return v[..., : self.dims[2]]
 
     @property
     def used_axes(self) -> list[int]:
@@ -125,6 +144,7 @@ class DevicePlan:
         d = nat.NwDesc()
         d.ndim, d.order, d.absorbing = host.ndim, host.order, host.absorbing
         d.dtype = nat.NW_F64 if dtype == torch.float64 else nat.NW_F32
+        d.pitch = host.row_pitch
         for a in range(3):
             d.dims[a], d.n_abs[a], d.field_dims[a] = host.dims[a], host.n_abs[a], host.field_dims[a]
             d.h[a] = host.h[a]
@@ -137,7 +157,7 @@ class DevicePlan:
         d.n_steps, d.n_t, d.substeps, d.n_shots = host.n_steps, host.n_t, host.substeps, host.n_shots
         if host.src_idx is not None and host.src_idx.numel():
             d.src_taps = host.src_idx.shape[1]
-            d.src_idx = dev(host.src_idx, torch.int32)
+            d.src_idx = dev(host.pitched(host.src_idx), torch.int32)
             d.src_val = dev(host.src_val)
             d.wavelet = dev(host.wavelet)
         d.rec_taps = 1 << host.ndim
@@ -147,10 +167,11 @@ class DevicePlan:
             d.rec_first = dev(host.rec_first, torch.int32)
             d.rec_count = dev(host.rec_count, torch.int32)
             d.trace_first = dev(host.trace_first, torch.int32)
-            d.rec_idx = dev(host.rec_idx, torch.int32)
+            ridx = host.pitched(host.rec_idx)
+            d.rec_idx = dev(ridx, torch.int32)
             d.rec_w = dev(host.rec_w)
             groups = list(zip(host.rec_first.tolist(), host.rec_count.tolist()))
-            f, n, cell, off, rec, w = injection_tables(host.rec_idx, host.rec_w, groups)
+            f, n, cell, off, rec, w = injection_tables(ridx, host.rec_w, groups)
             d.inj_max = int(n.max()) if n.numel() else 0
             d.inj_first, d.inj_count = dev(f, torch.int32), dev(n, torch.int32)
             d.inj_cell, d.inj_off = dev(cell, torch.int32), dev(off, torch.int32)
@@ -187,6 +208,15 @@ class DevicePlan:
     def set_tiles(self, kind: int, rows: int, shots_per_group: int) -> None:
         nat.check(nat.lib().nw_plan_set_tiles(self.handle, kind, rows, shots_per_group),
                   "nw_plan_set_tiles")
+
+    def set_option(self, option: int, value: int) -> None:
+        nat.check(nat.lib().nw_plan_set_option(self.handle, option, value), "nw_plan_set_option")
+
+    def option(self, option: int) -> int:
+        v = ctypes.c_int32()
+        nat.check(nat.lib().nw_plan_get_option(self.handle, option, ctypes.byref(v)),
+                  "nw_plan_get_option")
+        return v.value
 
     def grad_slices(self) -> int:
         return int(nat.lib().nw_grad_slices(self.handle))

@@ -98,16 +98,31 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
         choice = table[key]
     else:
         steps = min(steps, max(1, h.n_steps))
-        choice = {}
-        for kind, name in ((0, "step"), (1, "adjoint")):
-            timings = {}
-            for rows, spw in candidates(h.n_shots):
-                _time_kind(plan, kind, rows, spw, 1)  # first launch of a tile compiles nothing
-                timings[(rows, spw)] = _time_kind(plan, kind, rows, spw, steps)
-            (rows, spw), best = min(timings.items(), key=lambda kv: kv[1])
-            choice[name] = {"rows": rows, "shots_per_group": spw, "ms": best}
+        best_total, choice = None, None
+        for xcd in (1, 0):
+            plan.set_option(nat.OPT_XCD_MAP, xcd)
+            pick = {"xcd_map": xcd, "table": {}}
+            for kind, name in ((0, "step"), (1, "adjoint")):
+                timings = {}
+                for rpi in ((1, 2) if kind == 0 else (1,)):
+                    plan.set_option(nat.OPT_ROWS_PER_ITER, rpi)
+                    for rows, spw in candidates(h.n_shots):
+                        _time_kind(plan, kind, rows, spw, 1)  # warm the tile once
+                        timings[(rows, spw, rpi)] = _time_kind(plan, kind, rows, spw, steps)
+                (rows, spw, rpi), best = min(timings.items(), key=lambda kv: kv[1])
+                pick[name] = {"rows": rows, "shots_per_group": spw, "ms": best}
+                if kind == 0:
+                    pick["rows_per_iter"] = rpi
+                pick["table"][name] = {f"{r}x{g}/{u}": round(t, 4)
+                                       for (r, g, u), t in timings.items()}
+            # a gradient launches about two forward steps (sweep + recompute) per reverse step
+            total = 2.0 * pick["step"]["ms"] + pick["adjoint"]["ms"]
+            if best_total is None or total < best_total:
+                best_total, choice = total, pick
         table[key] = choice
         _store(table)
+    plan.set_option(nat.OPT_XCD_MAP, int(choice.get("xcd_map", 1)))
+    plan.set_option(nat.OPT_ROWS_PER_ITER, int(choice.get("rows_per_iter", 1)))
     plan.set_tiles(0, choice["step"]["rows"], choice["step"]["shots_per_group"])
     plan.set_tiles(1, choice["adjoint"]["rows"], choice["adjoint"]["shots_per_group"])
     return choice

@@ -452,6 +452,8 @@ class _LeapfrogBase(Operator):
             h=(1.0,) * lead + tuple(float(h) for h in g.spacing),
             dt=self.dt, n_steps=self.n_steps, n_t=self.n_t, substeps=self.substeps,
             n_shots=n_shots,
+            # wavefield rows start on 128-byte cache lines (32 elements of float32)
+            pitch=-(-g.padded_shape[-1] // 32) * 32,
         )
         dt = self.dt
         for ax in range(d):
@@ -513,7 +515,6 @@ class _LeapfrogBase(Operator):
         bad = [n for n in rec if n < 0 or n > n_steps]
         if bad:
             raise ConfigError(f"record indices {bad} outside [0, {n_steps}]")
-        shape = (h.n_shots, *self.grid.padded_shape)
         taper = None
         if h.absorbing == nat.NW_CERJAN:
             tap = None
@@ -526,10 +527,10 @@ class _LeapfrogBase(Operator):
             taper = tap.to(device=P.device, dtype=P.dtype)
 
         def snapshot(n):
-            prev = state[0].view(shape)
+            prev = h.wavefield_view(state[0], h.n_shots)
             if taper is not None and n > first_step:
                 prev = prev * taper
-            out = [prev, state[1].view(shape)]
+            out = [prev, h.wavefield_view(state[1], h.n_shots)]
             if P.pml:
                 for a, axis in enumerate(h.used_axes):
                     fs = list(self.grid.padded_shape)
@@ -875,7 +876,7 @@ def time_reversal(
     prop, lead, batch, dev, work, m, ma = op._setup(probe)
     P = prop.plan
     trd = tr.to(device=dev, dtype=work)
-    cells = g.nearest_index(op.receivers).to(device=dev, dtype=torch.int32).contiguous()
+    cells = P.host.pitched(g.nearest_index(op.receivers)).to(device=dev, dtype=torch.int32)
 
     def reversed_data(n: int) -> torch.Tensor:
         u = (n_t - 1) - n / sub
@@ -894,5 +895,5 @@ def time_reversal(
     for n in range(op.n_steps):
         P.forward(m, state, psi, n, n + 1)
         clamp(state[1], n + 1)
-    out = g.crop(state[1].view(*g.padded_shape))
+    out = g.crop(P.host.wavefield_view(state[1], 1)[0])
     return out.to(device=tr.device, dtype=dtype).clone()
