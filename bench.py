@@ -90,6 +90,21 @@ def cpu_baseline(wl, budget_s=15.0):
     }
 
 
+def measured_traffic(kernel_key: str):
+    """HBM bytes per launch of one kernel instance from the committed rocprofv3 counter runs
+    (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration kernel)."""
+    for path in sorted((ROOT / "profiles").glob("pmc_traffic_*.md")):
+        text = path.read_text()
+        start = text.rfind("```json")
+        if start < 0:
+            continue
+        blob = json.loads(text[start + 7 : text.index("```", start + 7)])
+        k = blob["kernels"].get(kernel_key)
+        if k:
+            return k["fetch_bytes"] + k["write_bytes"], path.name
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,16 +181,25 @@ def main() -> int:
              for kind, name in ((0, "step"), (1, "adjoint"))}
     tiles["xcd_map"] = plan.option(nat.OPT_XCD_MAP)
     tiles["rows_per_iter"] = plan.option(nat.OPT_ROWS_PER_ITER)
+    tiles["lds"] = [plan.option(nat.OPT_LDS_TILES_STEP), plan.option(nat.OPT_LDS_TILES_ADJOINT)]
     dom_kind = max(("step", "adjoint"), key=lambda k: timing[k][0])
     rows_t = tiles[dom_kind]["rows"]
-    kname = {"step": "k2_step" if rows_t else "k_step",
-             "adjoint": "k2_adjoint" if rows_t else "k_adjoint"}[dom_kind]
+    lds_t = tiles["lds"][0 if dom_kind == "step" else 1]
+    fam = "k3" if lds_t else "k2"
+    kname = {"step": f"{fam}_step" if rows_t else "k_step",
+             "adjoint": f"{fam}_adjoint" if rows_t else "k_adjoint"}[dom_kind]
     kname += f"<order 4, sponge{', rows %d' % rows_t if rows_t else ''}>"
     tot_ms, launches = timing[dom_kind]
     # (under a counter-collecting profiler the library's events record nothing)
     avg_s = tot_ms / 1e3 / launches if launches and tot_ms > 0 else float("nan")
     cells_per_launch = padded[0] * padded[1] * args.shots
     achieved = BYTES_PER_UPDATE[dom_kind] * cells_per_launch / avg_s / 1e9 if avg_s == avg_s else 0.0
+    r_code = 2 if op.order == 4 else 1
+    a_code = {"none": 0, "sponge": 1, "cerjan": 2, "pml": 3}[op.absorbing]
+    inst = f"nw::{kname.split('<')[0]}<{r_code}, {a_code}, {rows_t}"
+    if fam == "k2" and dom_kind == "step":
+        inst += f", {tiles['rows_per_iter']}"
+    traffic, traffic_src = measured_traffic(inst + ">")
     roofline = {
         "bound": "hbm",
         "kernel": kname,
@@ -183,7 +207,8 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "bytes_per_cell_update": BYTES_PER_UPDATE[dom_kind],
         "avg_launch_us": round(avg_s * 1e6, 2),
     }

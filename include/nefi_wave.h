@@ -166,8 +166,16 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
 /* Plan options (no counterpart in the reference; autotuner knobs).
  * NW_OPT_XCD_MAP: 1 (default) hands every XCD a contiguous run of the fast kernels' tiles so
  * neighbouring row strips share an L2; 0 numbers workgroups linearly.
- * NW_OPT_ROWS_PER_ITER: rows the fast forward step loads before computing (1 default, or 2). */
-enum nw_option { NW_OPT_XCD_MAP = 0, NW_OPT_ROWS_PER_ITER = 1 };
+ * NW_OPT_ROWS_PER_ITER: rows the fast forward step loads before computing (1 default, or 2).
+ * NW_OPT_LDS_TILES_STEP / _ADJOINT: 1 runs the LDS-tiled variant of the fast forward / reverse
+ *   kernel (whole tile with halo loaded at once; needs rows of 8, 16 or 32); 0 (default) the
+ *   register march. */
+enum nw_option {
+  NW_OPT_XCD_MAP = 0,
+  NW_OPT_ROWS_PER_ITER = 1,
+  NW_OPT_LDS_TILES_STEP = 2,
+  NW_OPT_LDS_TILES_ADJOINT = 3
+};
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
 

@@ -164,6 +164,8 @@ def lib() -> ctypes.CDLL:
 K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT = 0, 1, 2, 3
 OPT_XCD_MAP = 0
 OPT_ROWS_PER_ITER = 1
+OPT_LDS_TILES_STEP = 2
+OPT_LDS_TILES_ADJOINT = 3
 
 
 def profile_enable(on: bool) -> None:

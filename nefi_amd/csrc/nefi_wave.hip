@@ -482,6 +482,7 @@ struct nw_plan {
   int spw[2];
   int xcd_map;  // NW_OPT_XCD_MAP
   int rows_per_iter;  // NW_OPT_ROWS_PER_ITER (forward step)
+  int lds_tiles[2];   // NW_OPT_LDS_TILES_STEP / _ADJOINT
   int64_t shot_bytes;          // one wavefield (all shots)
   int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
 };
@@ -592,7 +593,7 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
   if constexpr (std::is_same<T, float>::value) {
     if (P->rows[0] > 0) {
       fast2d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[0], P->spw[0],
-                  P->xcd_map | (P->rows_per_iter == 2 ? 2 : 0), st);
+                  P->xcd_map | (P->rows_per_iter == 2 ? 2 : 0) | (P->lds_tiles[0] ? 4 : 0), st);
       return;
     }
   }
@@ -614,7 +615,7 @@ static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
   if constexpr (std::is_same<T, float>::value) {
     if (P->rows[1] > 0) {
       fast2d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[1], P->spw[1],
-                     P->xcd_map, st);
+                     P->xcd_map | (P->lds_tiles[1] ? 4 : 0), st);
       return;
     }
   }
@@ -932,6 +933,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   }
   p->xcd_map = 1;
   p->rows_per_iter = 1;
+  p->lds_tiles[0] = p->lds_tiles[1] = 0;
   const size_t es = d.dtype == NW_F64 ? 8 : 4;
   p->shot_bytes = cells * (int64_t)es * d.n_shots;
   for (int a = 0; a < 3; ++a)
@@ -957,6 +959,7 @@ int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_
   if (rows == 0) {
     P->rows[kind] = 0;
     P->spw[kind] = 1;
+    P->lds_tiles[kind] = 0;
     return NW_OK;
   }
   if (!fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
@@ -967,6 +970,7 @@ int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_
     return fail(NW_ERR_ARG, "nw_plan_set_tiles: shots_per_group must be in [1, n_shots]");
   P->rows[kind] = rows;
   P->spw[kind] = shots_per_group;
+  if (rows < 8 || rows > 32) P->lds_tiles[kind] = 0;
   return NW_OK;
 }
 
@@ -989,6 +993,13 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     P->rows_per_iter = value;
     return NW_OK;
   }
+  if (option == NW_OPT_LDS_TILES_STEP || option == NW_OPT_LDS_TILES_ADJOINT) {
+    const int k = option == NW_OPT_LDS_TILES_STEP ? 0 : 1;
+    if (value && (P->rows[k] < 8 || P->rows[k] > 32))
+      return fail(NW_ERR_ARG, "LDS tiles need rows of 8, 16 or 32");
+    P->lds_tiles[k] = value != 0;
+    return NW_OK;
+  }
   return fail(NW_ERR_ARG, "nw_plan_set_option: unknown option");
 }
 
@@ -1000,6 +1011,10 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_ROWS_PER_ITER) {
     *value = P->rows_per_iter;
+    return NW_OK;
+  }
+  if (option == NW_OPT_LDS_TILES_STEP || option == NW_OPT_LDS_TILES_ADJOINT) {
+    *value = P->lds_tiles[option == NW_OPT_LDS_TILES_STEP ? 0 : 1];
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

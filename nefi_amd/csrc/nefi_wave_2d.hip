@@ -367,6 +367,249 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
 }
 
 // ------------------------------------------------------------------------------------------
+// LDS-tiled variants: one workgroup = one tile of TY rows x 256 columns. Every tile load (with
+// its R-row halo and 4-column margins) is issued at once, so a tile costs one memory round
+// trip and its halo is read once; wavefront w then marches rows w*TY/4 .. (w+1)*TY/4-1 of the
+// tile out of LDS. Rows of one tile are fully unrolled, so per-row values stay in registers.
+// ------------------------------------------------------------------------------------------
+constexpr int kLW = 256 + 8;  // LDS row: 4 margin columns on each side, 16-byte aligned
+
+template <int TY, int R>
+struct TileShape {
+  static constexpr int rows = TY + 2 * R;
+  static constexpr int n4 = rows * (kLW / 4);  // float4 elements of one tile
+  static constexpr int per_thread = (n4 + 255) / 256;
+  static constexpr int my_rows = TY / 4;
+};
+
+// issue every float4 load of a tile (rows y0-R .. y0+TY+R-1, columns x0-4 .. x0+259)
+template <int TY, int R>
+__device__ __forceinline__ void tile_load(const float* g, int y0, int x0, int D1, int D2, int P2,
+                                          float4* v) {
+  using S = TileShape<TY, R>;
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+    const int gy = y0 - R + row, gx = x0 - 4 + 4 * c4;
+    const bool ok = i < S::n4 && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
+    v[k] = ok ? ld4(g + (gy * P2 + gx)) : f4(0.f);
+  }
+}
+
+template <int TY, int R>
+__device__ __forceinline__ void tile_store(float* lds, const float4* v) {
+  using S = TileShape<TY, R>;
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < S::n4) *reinterpret_cast<float4*>(lds + 4 * i) = v[k];
+  }
+}
+
+// row r of the LDS tile at the lane's four columns, and its x-neighbours (x-2, x-1, x+4, x+5)
+__device__ __forceinline__ float4 lrow(const float* lds, int r, int lane) {
+  return *reinterpret_cast<const float4*>(lds + r * kLW + 4 * lane + 4);
+}
+__device__ __forceinline__ float4 lnb(const float* lds, int r, int lane) {
+  const float2 a = *reinterpret_cast<const float2*>(lds + r * kLW + 4 * lane + 2);
+  const float2 b = *reinterpret_cast<const float2*>(lds + r * kLW + 4 * lane + 8);
+  return make_float4(a.x, a.y, b.x, b.y);
+}
+
+__device__ __forceinline__ Tile tile_of_rows(int nX, int nY, int nG, int xcd_map) {
+  return tile_of(nX, nY, nG, xcd_map);
+}
+
+template <int R, int ABS, int TY>
+__global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, int shots,
+                                               int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lp[S::rows * kLW];
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;  // first tile row of this wavefront
+  const bool act = x < D2;
+  const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
+  // model rows of this wavefront: shot-independent, loaded once for the group
+  float4 mm[S::my_rows];
+#pragma unroll
+  for (int k = 0; k < S::my_rows; ++k) {
+    const int y = y0 + r0 + k;
+    mm[k] = (act && y < D1) ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+  }
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
+    const int64_t base = (int64_t)s * g.cells;
+    const float* P = io.p + base;
+    float4 v[S::per_thread];
+    tile_load<TY, R>(P, y0, x0, D1, D2, P2, v);
+    float4 pp[S::my_rows];
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      pp[k] = (act && y < D1) ? ld4(io.p_prev + base + (y * P2 + x)) : f4(0.f);
+    }
+    __syncthreads();  // the previous shot's reads of lp are done
+    tile_store<TY, R>(lp, v);
+    __syncthreads();
+    float4 w[2 * R + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) w[k] = lrow(lp, r0 + k, lane);
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      w[2 * R] = lrow(lp, r0 + k + 2 * R, lane);
+      if (act && y < D1) {
+        const float4 nb = lnb(lp, r0 + k + R, lane);
+        const int ro = y * P2 + x;
+        const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+        const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+        const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+        const float4 pc = w[R];
+        const int tpv = io.taper_prev;
+        float4 o = make_float4(rhs1<ABS>(pc.x, pp[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
+                               rhs1<ABS>(pc.y, pp[k].y, mm[k].y, lap.y, dm.b.y, dm.tap.y, tpv),
+                               rhs1<ABS>(pc.z, pp[k].z, mm[k].z, lap.z, dm.b.z, dm.tap.z, tpv),
+                               rhs1<ABS>(pc.w, pp[k].w, mm[k].w, lap.w, dm.b.w, dm.tap.w, tpv));
+        for (int t = 0; t < io.src_taps; ++t) {
+          const int d = io.src_idx[s * io.src_taps + t] - ro;
+          if (d >= 0 && d < 4) {
+            const float sv = io.src_val[s * io.src_taps + t] * w_n;
+            o.x += d == 0 ? sv : 0.f;
+            o.y += d == 1 ? sv : 0.f;
+            o.z += d == 2 ? sv : 0.f;
+            o.w += d == 3 ? sv : 0.f;
+          }
+        }
+        if (ABS != NW_NONE) o = mul4(dm.a, o);
+        st4(io.p_next + base + ro, o);
+        if (io.ck_prev) {
+          st4(io.ck_prev + base + ro, pp[k]);
+          st4(io.ck_cur + base + ro, pc);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) w[j] = w[j + 1];
+    }
+  }
+}
+
+template <int R, int ABS, int TY>
+__global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io, int shots,
+                                                  int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
+  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;
+  const bool act = x < D2;
+  const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const bool imaging = io.p_n != nullptr && io.grad != nullptr;
+  const int tpn = io.taper_next;
+  // float64 imaging sums of this wavefront's cells over the shot group
+  double2 ga[S::my_rows], gb[S::my_rows];
+#pragma unroll
+  for (int k = 0; k < S::my_rows; ++k) {
+    ga[k] = make_double2(0.0, 0.0);
+    gb[k] = ga[k];
+  }
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
+    const int64_t base = (int64_t)s * g.cells;
+    const float* L1 = io.lam1 + base;
+    {
+      float4 vl[S::per_thread], vm[S::per_thread];
+      tile_load<TY, R>(L1, y0, x0, D1, D2, P2, vl);
+      tile_load<TY, R>(io.ma, y0, x0, D1, D2, P2, vm);
+      __syncthreads();  // the previous shot's reads of lq / lp are done
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
+      tile_store<TY, R>(lq, vl);
+    }
+    if (imaging) {
+      float4 vp[S::per_thread];
+      tile_load<TY, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
+      tile_store<TY, R>(lp, vp);
+    }
+    float4 lc[S::my_rows], l2c[S::my_rows];
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      const bool ok = act && y < D1;
+      lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // just loaded into the tile: cache hit
+      l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+    }
+    __syncthreads();
+    float4 qw[2 * R + 1], pw[2 * R + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) {
+      qw[k] = lrow(lq, r0 + k, lane);
+      pw[k] = imaging ? lrow(lp, r0 + k, lane) : f4(0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
+      pw[2 * R] = imaging ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
+      if (act && y < D1) {
+        const float4 qn = lnb(lq, r0 + k + R, lane);
+        const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
+        const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+        const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+        float4 rb;
+        const float4 o = make_float4(
+            adj1<ABS>(lc[k].x, l2c[k].x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
+            adj1<ABS>(lc[k].y, l2c[k].y, lapq.y, dm.a.y, dm.b.y, dm.tap.y, tpn, rb.y),
+            adj1<ABS>(lc[k].z, l2c[k].z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
+            adj1<ABS>(lc[k].w, l2c[k].w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
+        st4(io.lam0 + base + (y * P2 + x), o);
+        if (imaging) {
+          const float4 pn = lnb(lp, r0 + k + R, lane);
+          const float4 lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
+          ga[k].x += (double)rb.x * (double)lapp.x;
+          ga[k].y += (double)rb.y * (double)lapp.y;
+          gb[k].x += (double)rb.z * (double)lapp.z;
+          gb[k].y += (double)rb.w * (double)lapp.w;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) {
+        qw[j] = qw[j + 1];
+        pw[j] = pw[j + 1];
+      }
+    }
+  }
+  if (imaging) {
+    double* G = io.grad + (int64_t)tl.grp * g.cells;
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      if (act && y < D1) {
+        double2* gp = reinterpret_cast<double2*>(G + (y * P2 + x));
+        double2 a = gp[0], b = gp[1];
+        a.x += ga[k].x;
+        a.y += ga[k].y;
+        b.x += gb[k].x;
+        b.y += gb[k].y;
+        gp[0] = a;
+        gp[1] = b;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype) {
@@ -379,11 +622,20 @@ static unsigned grid_tiles(const Geo<float>& g, int ty, int shots, int spw) {
   return (unsigned)(kXcds * div_up(T, kXcds));
 }
 
+static unsigned grid_tiles3(const Geo<float>& g, int ty, int shots, int spw) {
+  const int T = div_up((int)g.D[2], 256) * div_up((int)g.D[1], ty) * div_up(shots, spw);
+  return (unsigned)(kXcds * div_up(T, kXcds));
+}
+
 template <int R, int ABS, int TY>
 static void launch_step_t(const Geo<float>& g, const StepIO<float>& io, int shots, int spw,
                           int xm, hipStream_t st) {
-  // xm bit 1 selects the two-rows-per-iteration march
-  if (xm & 2)
+  // xm bit 2 selects the LDS-tiled kernel, bit 1 the two-rows-per-iteration march
+  if ((xm & 4) && TY >= 8 && TY <= 32)
+    hipLaunchKernelGGL((k3_step<R, ABS, (TY < 8 ? 8 : (TY > 32 ? 32 : TY))>),
+                       dim3(grid_tiles3(g, TY, shots, spw)), dim3(256), 0, st, g, io, shots, spw,
+                       xm & 1);
+  else if (xm & 2)
     hipLaunchKernelGGL((k2_step<R, ABS, TY, 2>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);
   else
@@ -394,8 +646,13 @@ static void launch_step_t(const Geo<float>& g, const StepIO<float>& io, int shot
 template <int R, int ABS, int TY>
 static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int spw, int xm,
                          hipStream_t st) {
-  hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256), 0,
-                     st, g, io, shots, spw, xm & 1);
+  if ((xm & 4) && TY >= 8 && TY <= 32)
+    hipLaunchKernelGGL((k3_adjoint<R, ABS, (TY < 8 ? 8 : (TY > 32 ? 32 : TY))>),
+                       dim3(grid_tiles3(g, TY, shots, spw)), dim3(256), 0, st, g, io, shots, spw,
+                       xm & 1);
+  else
+    hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
+                       0, st, g, io, shots, spw, xm & 1);
 }
 
 template <int R, int ABS>

@@ -21,9 +21,11 @@ from pathlib import Path
 import torch
 
 from .. import _native as nat
+from ..errors import NativeError
 
 ROWS = (4, 8, 16, 32, 64)
-GROUPS = (1, 2, 4, 8)
+LDS_ROWS = (8, 16, 32)
+GROUPS = (1, 2, 4, 8, 16)
 # below this many cell-updates per step a launch is latency-bound and every tile ties
 MIN_WORK = 1 << 18
 
@@ -57,14 +59,25 @@ def plan_key(plan) -> str:
     return f"{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
 
 
-def candidates(n_shots: int):
-    return [(r, g) for r in ROWS for g in GROUPS if g <= n_shots]
+def candidates(kind: int, n_shots: int):
+    """(rows, shots per group, rows loaded per iteration, LDS tiles) to try for a kernel."""
+    groups = [g for g in GROUPS if g <= n_shots]
+    out = [(r, g, u, 0) for r in ROWS for g in groups for u in ((1, 2) if kind == 0 else (1,))]
+    out += [(r, g, 1, 1) for r in LDS_ROWS for g in groups]
+    return out
 
 
-def _time_kind(plan, kind: int, rows: int, spw: int, steps: int) -> float:
-    """Average launch time (ms) of one kernel kind at one tile."""
+def _apply(plan, kind: int, rows: int, spw: int, rpi: int, lds: int) -> None:
+    plan.set_tiles(kind, rows, spw)
+    if kind == 0:
+        plan.set_option(nat.OPT_ROWS_PER_ITER, rpi)
+    plan.set_option(nat.OPT_LDS_TILES_STEP if kind == 0 else nat.OPT_LDS_TILES_ADJOINT, lds)
+
+
+def _time_kind(plan, kind: int, cand, steps: int) -> float:
+    """Average launch time (ms) of one kernel kind at one candidate."""
     P = plan
-    P.set_tiles(kind, rows, spw)
+    _apply(P, kind, *cand)
     cells = P.host.cells
     m = P.zeros(cells)
     state, psi = P.new_state()
@@ -104,17 +117,17 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
             pick = {"xcd_map": xcd, "table": {}}
             for kind, name in ((0, "step"), (1, "adjoint")):
                 timings = {}
-                for rpi in ((1, 2) if kind == 0 else (1,)):
-                    plan.set_option(nat.OPT_ROWS_PER_ITER, rpi)
-                    for rows, spw in candidates(h.n_shots):
-                        _time_kind(plan, kind, rows, spw, 1)  # warm the tile once
-                        timings[(rows, spw, rpi)] = _time_kind(plan, kind, rows, spw, steps)
-                (rows, spw, rpi), best = min(timings.items(), key=lambda kv: kv[1])
-                pick[name] = {"rows": rows, "shots_per_group": spw, "ms": best}
-                if kind == 0:
-                    pick["rows_per_iter"] = rpi
-                pick["table"][name] = {f"{r}x{g}/{u}": round(t, 4)
-                                       for (r, g, u), t in timings.items()}
+                for cand in candidates(kind, h.n_shots):
+                    try:
+                        _time_kind(plan, kind, cand, 1)  # warm the candidate once
+                        timings[cand] = _time_kind(plan, kind, cand, steps)
+                    except NativeError:  # a variant this device refuses to launch
+                        nat.profile_enable(False)
+                (rows, spw, rpi, lds), best = min(timings.items(), key=lambda kv: kv[1])
+                pick[name] = {"rows": rows, "shots_per_group": spw, "rows_per_iter": rpi,
+                              "lds": lds, "ms": best}
+                pick["table"][name] = {f"{r}x{g}/{u}{'L' if l else ''}": round(t, 4)
+                                       for (r, g, u, l), t in timings.items()}
             # a gradient launches about two forward steps (sweep + recompute) per reverse step
             total = 2.0 * pick["step"]["ms"] + pick["adjoint"]["ms"]
             if best_total is None or total < best_total:
@@ -122,7 +135,8 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
         table[key] = choice
         _store(table)
     plan.set_option(nat.OPT_XCD_MAP, int(choice.get("xcd_map", 1)))
-    plan.set_option(nat.OPT_ROWS_PER_ITER, int(choice.get("rows_per_iter", 1)))
-    plan.set_tiles(0, choice["step"]["rows"], choice["step"]["shots_per_group"])
-    plan.set_tiles(1, choice["adjoint"]["rows"], choice["adjoint"]["shots_per_group"])
+    for kind, name in ((0, "step"), (1, "adjoint")):
+        c = choice[name]
+        _apply(plan, kind, c["rows"], c["shots_per_group"], c.get("rows_per_iter", 1),
+               c.get("lds", 0))
     return choice

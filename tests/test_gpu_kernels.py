@@ -40,7 +40,9 @@ def _survey(absorbing="sponge", order=4, n=(70, 300), shots=5, n_t=60):
     return Domain(n, ext), geo, c, kw
 
 
-def _grad(op, c, w, tiles=None):
+def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1):
+    from nefi_amd import _native as nat
+
     x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
     y = op({"c": x})
     if tiles is not None:
@@ -48,6 +50,9 @@ def _grad(op, c, w, tiles=None):
         prop = next(iter(op._cache.values()))
         for kind, (rows, spw) in enumerate(tiles):
             prop.plan.set_tiles(kind, rows, spw)
+        prop.plan.set_option(nat.OPT_LDS_TILES_STEP, lds[0])
+        prop.plan.set_option(nat.OPT_LDS_TILES_ADJOINT, lds[1])
+        prop.plan.set_option(nat.OPT_XCD_MAP, xcd)
         y = op({"c": x})
     (g,) = torch.autograd.grad((y * w).sum(), x)
     return y.detach().cpu().double().numpy(), g.cpu().double().numpy()
@@ -63,10 +68,14 @@ def test_fast_kernels_match_generic_over_tiles(absorbing, order):
     w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
                     generator=torch.Generator(DEV).manual_seed(0))
     y0, g0 = _grad(op, c, w, tiles=[(0, 1), (0, 1)])
-    for tiles in ([(16, 1), (16, 1)], [(4, 2), (8, 3)], [(32, 5), (64, 5)], [(8, 1), (4, 4)]):
-        y, g = _grad(op, c, w, tiles=tiles)
+    for tiles, lds, xcd in (([(16, 1), (16, 1)], (0, 0), 1), ([(4, 2), (8, 3)], (0, 0), 0),
+                            ([(32, 5), (64, 5)], (0, 0), 1), ([(8, 1), (4, 4)], (0, 0), 1),
+                            ([(8, 2), (8, 3)], (1, 1), 1), ([(16, 1), (32, 5)], (1, 1), 0),
+                            ([(32, 4), (16, 2)], (1, 1), 1)):
+        y, g = _grad(op, c, w, tiles=tiles, lds=lds, xcd=xcd)
         ey, eg = rel(y, y0), rel(g, g0)
-        print(f"{absorbing} o{order} tiles {tiles}: traces {ey:.2e}, gradient {eg:.2e}", flush=True)
+        print(f"{absorbing} o{order} tiles {tiles} lds {lds} xcd {xcd}: traces {ey:.2e}, "
+              f"gradient {eg:.2e}", flush=True)
         assert ey < 1e-6 and eg < 1e-5
 
 
