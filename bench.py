@@ -161,6 +161,9 @@ def main() -> int:
         step()
         torch.cuda.synchronize()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+    kinds = {"step": nat.K_STEP, "adjoint": nat.K_ADJOINT, "sample": nat.K_SAMPLE,
+             "inject": nat.K_INJECT}
+    # timed region 1: every kernel launch bracketed by HIP events (calls run eagerly)
     nat.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -169,19 +172,30 @@ def main() -> int:
         print(f"[bench] step {i} queued", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kinds = {"step": nat.K_STEP, "adjoint": nat.K_ADJOINT, "sample": nat.K_SAMPLE,
-             "inject": nat.K_INJECT}
     timing = {k: nat.profile_read(v) for k, v in kinds.items()}
     nat.profile_enable(False)
+    # timed region 2: the same gradients replayed from the captured hipGraphs (no events)
+    plan = next(iter(op._cache.values())).plan
+    graph_ms = None
+    if plan.option(nat.OPT_GRAPH):
+        step()  # captures any call whose graph is not cached yet
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        print(f"[bench] hipGraph replays: {graph_ms:.1f} ms per gradient, "
+              f"{plan.option(nat.OPT_GRAPH_COUNT)} graphs", file=sys.stderr, flush=True)
     ms_per_step = elapsed * 1e3 / args.steps
     value = updates / (elapsed / args.steps) / 1e9
 
-    plan = next(iter(op._cache.values())).plan
     tiles = {name: dict(zip(("rows", "shots_per_group"), plan.tiles(kind)))
              for kind, name in ((0, "step"), (1, "adjoint"))}
     tiles["xcd_map"] = plan.option(nat.OPT_XCD_MAP)
     tiles["rows_per_iter"] = plan.option(nat.OPT_ROWS_PER_ITER)
     tiles["lds"] = [plan.option(nat.OPT_LDS_TILES_STEP), plan.option(nat.OPT_LDS_TILES_ADJOINT)]
+    tiles["hip_graph"] = plan.option(nat.OPT_GRAPH)
     dom_kind = max(("step", "adjoint"), key=lambda k: timing[k][0])
     rows_t = tiles[dom_kind]["rows"]
     lds_t = tiles["lds"][0 if dom_kind == "step" else 1]
@@ -223,6 +237,8 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 2),
+        "value_hip_graph": round(updates / (graph_ms / 1e3) / 1e9, 3) if graph_ms else None,
+        "ms_per_step_hip_graph": round(graph_ms, 2) if graph_ms else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -169,12 +169,21 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  * NW_OPT_ROWS_PER_ITER: rows the fast forward step loads before computing (1 default, or 2).
  * NW_OPT_LDS_TILES_STEP / _ADJOINT: 1 runs the LDS-tiled variant of the fast forward / reverse
  *   kernel (whole tile with halo loaded at once; needs rows of 8, 16 or 32); 0 (default) the
- *   register march. */
+ *   register march.
+ * NW_OPT_GRAPH: 1 captures every distinct nw_forward / nw_adjoint call (same buffers, step
+ *   range and flags) into a hipGraph on a private stream the first time and replays it on the
+ *   caller's stream afterwards; 0 (default) launches the kernels one by one (graphs captured
+ *   so far are kept for when it is switched back on). Changing tiles or the other options
+ *   drops the captured graphs. The whole time loop of a call is one graph launch.
+ *   While kernel timing (nw_profile_enable) is on, calls run eagerly.
+ * NW_OPT_GRAPH_COUNT (read only): graphs the plan currently holds. */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,
   NW_OPT_LDS_TILES_STEP = 2,
-  NW_OPT_LDS_TILES_ADJOINT = 3
+  NW_OPT_LDS_TILES_ADJOINT = 3,
+  NW_OPT_GRAPH = 4,
+  NW_OPT_GRAPH_COUNT = 5
 };
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
@@ -236,10 +245,11 @@ int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* value
 
 /* Kernel timing (no counterpart in the reference; measurement support for benchmarks).
  * While enabled, every launch of the step, reverse-step, sample and inject kernels is bracketed
- * by HIP events on its own stream. nw_profile_read waits for the recorded events and returns
- * the summed duration (ms) and launch count of one kernel kind:
- * 0 = forward step, 1 = reverse step, 2 = receiver sample, 3 = residual inject.
- * Enabling clears earlier records. Process-wide; not thread-safe. */
+ * by HIP events on its own stream, and calls run eagerly even in graph mode. nw_profile_read
+ * waits for the recorded events and returns the summed duration (ms) and launch count of one
+ * kernel kind since profiling was enabled: 0 = forward step, 1 = reverse step,
+ * 2 = receiver sample, 3 = residual inject. Enabling clears earlier records. Process-wide; not
+ * thread-safe. */
 enum nw_kernel_kind { NW_K_STEP = 0, NW_K_ADJOINT = 1, NW_K_SAMPLE = 2, NW_K_INJECT = 3 };
 int32_t nw_profile_enable(int32_t on);
 int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches);

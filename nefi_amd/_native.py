@@ -166,6 +166,8 @@ OPT_XCD_MAP = 0
 OPT_ROWS_PER_ITER = 1
 OPT_LDS_TILES_STEP = 2
 OPT_LDS_TILES_ADJOINT = 3
+OPT_GRAPH = 4
+OPT_GRAPH_COUNT = 5
 
 
 def profile_enable(on: bool) -> None:

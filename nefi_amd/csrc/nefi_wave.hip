@@ -67,6 +67,14 @@ struct Profiler {
 
 static Profiler g_prof;
 
+// One captured call: its argument key, the instantiated graph and the pointer arrangement the
+// call leaves behind.
+struct GraphEntry {
+  std::vector<uintptr_t> key;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> out;
+};
+
 // brackets one launch; returns the record index or -1 when timing is off
 static int prof_begin(int kind, hipStream_t st) {
   if (!g_prof.on) return -1;
@@ -483,6 +491,11 @@ struct nw_plan {
   int xcd_map;  // NW_OPT_XCD_MAP
   int rows_per_iter;  // NW_OPT_ROWS_PER_ITER (forward step)
   int lds_tiles[2];   // NW_OPT_LDS_TILES_STEP / _ADJOINT
+  // hipGraph capture of whole nw_forward / nw_adjoint calls (NW_OPT_GRAPH)
+  int graph_mode;
+  hipStream_t capture_stream;
+  std::vector<nw::GraphEntry> graphs;
+  int64_t graph_serial;
   int64_t shot_bytes;          // one wavefield (all shots)
   int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
 };
@@ -854,6 +867,58 @@ static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** 
   return NW_OK;
 }
 
+// plans alive in the process (for timing reads across their captured graphs)
+std::vector<nw_plan*> g_plans;
+
+static void drop_graphs(nw_plan* P) {
+  for (auto& e : P->graphs)
+    if (e.exec) hipGraphExecDestroy(e.exec);
+  P->graphs.clear();
+}
+
+// Run `body(stream)` directly, or (graph mode) capture it once per distinct key on the plan's
+// capture stream and replay the instantiated graph on `st`. `io` holds the pointer arrays the
+// call rearranges; replays restore the arrangement recorded at capture.
+template <typename F>
+static int run_captured(nw_plan* P, std::vector<uintptr_t> key, std::vector<void**> io_arrays,
+                        std::vector<int> io_sizes, hipStream_t st, F body) {
+  // timed calls run eagerly: events recorded inside a graph cannot be read back here
+  if (!P->graph_mode || g_prof.on) return body(st);
+  for (size_t a = 0; a < io_arrays.size(); ++a)
+    for (int i = 0; i < io_sizes[a]; ++i) key.push_back((uintptr_t)io_arrays[a][i]);
+  GraphEntry* hit = nullptr;
+  for (auto& e : P->graphs)
+    if (e.key == key) hit = &e;
+  if (!hit) {
+    if (!P->capture_stream) NW_HIP(hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking));
+    if (P->graphs.size() >= 64) drop_graphs(P);
+    NW_HIP(hipStreamBeginCapture(P->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body(P->capture_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(P->capture_stream, &graph);
+    if (rc != NW_OK || ec != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      if (rc != NW_OK) return rc;
+      return fail(NW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    }
+    GraphEntry e;
+    e.key = key;
+    const hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+      return fail(NW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    for (size_t a = 0; a < io_arrays.size(); ++a)
+      for (int i = 0; i < io_sizes[a]; ++i) e.out.push_back(io_arrays[a][i]);
+    P->graphs.push_back(std::move(e));
+    hit = &P->graphs.back();
+  }
+  NW_HIP(hipGraphLaunch(hit->exec, st));
+  size_t k = 0;
+  for (size_t a = 0; a < io_arrays.size(); ++a)
+    for (int i = 0; i < io_sizes[a]; ++i) io_arrays[a][i] = hit->out[k++];
+  return NW_OK;
+}
+
 }  // namespace nw
 
 // ------------------------------------------------------------------------------------------
@@ -875,14 +940,16 @@ int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches) {
   if (!total_ms || !launches) return fail(NW_ERR_ARG, "nw_profile_read: null argument");
   double ms = 0.0;
   int64_t n = 0;
-  for (auto& r : g_prof.recs) {
-    if (r.kind != kind) continue;
+  auto add = [&](const ProfRec& r, int64_t times) -> int {
     NW_HIP(hipEventSynchronize(r.stop));
     float t = 0.f;
     NW_HIP(hipEventElapsedTime(&t, r.start, r.stop));
-    ms += t;
-    ++n;
-  }
+    ms += (double)t * (double)times;
+    n += times;
+    return NW_OK;
+  };
+  for (auto& r : g_prof.recs)
+    if (r.kind == kind) NW_TRY(add(r, 1));
   *total_ms = ms;
   *launches = n;
   return NW_OK;
@@ -934,6 +1001,10 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->xcd_map = 1;
   p->rows_per_iter = 1;
   p->lds_tiles[0] = p->lds_tiles[1] = 0;
+  p->graph_mode = 0;
+  p->capture_stream = nullptr;
+  p->graph_serial = 0;
+  nw::g_plans.push_back(p);
   const size_t es = d.dtype == NW_F64 ? 8 : 4;
   p->shot_bytes = cells * (int64_t)es * d.n_shots;
   for (int a = 0; a < 3; ++a)
@@ -945,6 +1016,14 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
 }
 
 int32_t nw_plan_destroy(nw_plan* plan) {
+  if (!plan) return NW_OK;
+  drop_graphs(plan);
+  if (plan->capture_stream) hipStreamDestroy(plan->capture_stream);
+  for (size_t i = 0; i < nw::g_plans.size(); ++i)
+    if (nw::g_plans[i] == plan) {
+      nw::g_plans.erase(nw::g_plans.begin() + i);
+      break;
+    }
   delete plan;
   return NW_OK;
 }
@@ -956,6 +1035,7 @@ int64_t nw_grad_slices(const nw_plan* P) {
 
 int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_per_group) {
   if (!P || kind < 0 || kind > 1) return fail(NW_ERR_ARG, "nw_plan_set_tiles: bad plan or kind");
+  drop_graphs(P);
   if (rows == 0) {
     P->rows[kind] = 0;
     P->spw[kind] = 1;
@@ -984,6 +1064,11 @@ int32_t nw_plan_get_tiles(const nw_plan* P, int32_t kind, int32_t* rows, int32_t
 
 int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
   if (!P) return fail(NW_ERR_ARG, "nw_plan_set_option: null plan");
+  if (option == NW_OPT_GRAPH) {  // keeps the graphs captured so far
+    P->graph_mode = value != 0;
+    return NW_OK;
+  }
+  drop_graphs(P);  // captured graphs bake the kernels the other options select
   if (option == NW_OPT_XCD_MAP) {
     P->xcd_map = value != 0;
     return NW_OK;
@@ -1005,6 +1090,14 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
 
 int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   if (!P || !value) return fail(NW_ERR_ARG, "nw_plan_get_option: null argument");
+  if (option == NW_OPT_GRAPH) {
+    *value = P->graph_mode;
+    return NW_OK;
+  }
+  if (option == NW_OPT_GRAPH_COUNT) {
+    *value = (int32_t)P->graphs.size();
+    return NW_OK;
+  }
   if (option == NW_OPT_XCD_MAP) {
     *value = P->xcd_map;
     return NW_OK;
@@ -1065,11 +1158,19 @@ int32_t nw_forward(nw_plan* P, const void* m, void** state, void** psi, int32_t 
   if (keep && keep_begin != n_begin) return fail(NW_ERR_ARG, "nw_forward: keep_begin must equal n_begin");
   if (ckpt && ckpt_every < 1) return fail(NW_ERR_ARG, "nw_forward: ckpt_every must be >= 1");
   hipStream_t st = (hipStream_t)stream;
-  if (P->d.dtype == NW_F32)
-    return forward_impl<float>(P, (const float*)m, state, psi, n_begin, n_end, (float*)traces,
-                               (float*)ckpt, ckpt_every, (float*)keep, keep_begin, first_step, st);
-  return forward_impl<double>(P, (const double*)m, state, psi, n_begin, n_end, (double*)traces,
-                              (double*)ckpt, ckpt_every, (double*)keep, keep_begin, first_step, st);
+  const int npsi = P->d.absorbing == NW_PML ? 2 * P->d.ndim : 0;
+  std::vector<uintptr_t> key = {0, (uintptr_t)m, (uintptr_t)n_begin, (uintptr_t)n_end,
+                                (uintptr_t)traces, (uintptr_t)ckpt, (uintptr_t)ckpt_every,
+                                (uintptr_t)keep, (uintptr_t)keep_begin, (uintptr_t)first_step};
+  return run_captured(P, key, {state, psi}, {2, npsi}, st, [&](hipStream_t s2) {
+    if (P->d.dtype == NW_F32)
+      return forward_impl<float>(P, (const float*)m, state, psi, n_begin, n_end, (float*)traces,
+                                 (float*)ckpt, ckpt_every, (float*)keep, keep_begin, first_step,
+                                 s2);
+    return forward_impl<double>(P, (const double*)m, state, psi, n_begin, n_end,
+                                (double*)traces, (double*)ckpt, ckpt_every, (double*)keep,
+                                keep_begin, first_step, s2);
+  });
 }
 
 int32_t nw_adjoint(nw_plan* P, const void* m, const void* ma, void** lam, void** psi_bar,
@@ -1083,13 +1184,19 @@ int32_t nw_adjoint(nw_plan* P, const void* m, const void* ma, void** lam, void**
   if (keep && keep_begin > n_begin) return fail(NW_ERR_ARG, "nw_adjoint: keep must start <= n_begin");
   if (grad_m && !keep) return fail(NW_ERR_ARG, "nw_adjoint: grad_m needs the kept wavefields");
   hipStream_t st = (hipStream_t)stream;
-  if (P->d.dtype == NW_F32)
-    return adjoint_impl<float>(P, (const float*)m, (const float*)ma, lam, psi_bar, n_begin, n_end,
-                               (const float*)keep, keep_begin, (const float*)residual, inject_end,
-                               grad_m, first_step, st);
-  return adjoint_impl<double>(P, (const double*)m, (const double*)ma, lam, psi_bar, n_begin,
-                              n_end, (const double*)keep, keep_begin, (const double*)residual,
-                              inject_end, grad_m, first_step, st);
+  const int npsi = P->d.absorbing == NW_PML ? 2 * P->d.ndim : 0;
+  std::vector<uintptr_t> key = {1, (uintptr_t)ma, (uintptr_t)n_begin, (uintptr_t)n_end,
+                                (uintptr_t)keep, (uintptr_t)keep_begin, (uintptr_t)residual,
+                                (uintptr_t)inject_end, (uintptr_t)grad_m, (uintptr_t)first_step};
+  return run_captured(P, key, {lam, psi_bar}, {2, npsi}, st, [&](hipStream_t s2) {
+    if (P->d.dtype == NW_F32)
+      return adjoint_impl<float>(P, (const float*)m, (const float*)ma, lam, psi_bar, n_begin,
+                                 n_end, (const float*)keep, keep_begin, (const float*)residual,
+                                 inject_end, grad_m, first_step, s2);
+    return adjoint_impl<double>(P, (const double*)m, (const double*)ma, lam, psi_bar, n_begin,
+                                n_end, (const double*)keep, keep_begin, (const double*)residual,
+                                inject_end, grad_m, first_step, s2);
+  });
 }
 
 int32_t nw_model_grad(nw_plan* P, const void* c, const double* grad_m, void* grad_c, void* stream) {

@@ -12,8 +12,10 @@ backward).
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
+import weakref
 from dataclasses import dataclass, field
 
 import torch
@@ -218,6 +220,16 @@ class DevicePlan:
                   "nw_plan_get_option")
         return v.value
 
+    @contextlib.contextmanager
+    def eager(self):
+        """Run calls without graph capture (calls whose buffers or step ranges never repeat)."""
+        was = self.option(nat.OPT_GRAPH)
+        self.set_option(nat.OPT_GRAPH, 0)
+        try:
+            yield self
+        finally:
+            self.set_option(nat.OPT_GRAPH, was)
+
     def grad_slices(self) -> int:
         return int(nat.lib().nw_grad_slices(self.handle))
 
@@ -323,6 +335,49 @@ def grad_block(plan: DevicePlan, n_steps: int, mode: str, every: int | None) -> 
     return k
 
 
+class Workspace:
+    """Named device buffers reused from call to call.
+
+    Reuse keeps buffer addresses stable, so the hipGraph the library captured for a call is
+    replayed by the next identical call instead of being captured again. A workspace is held
+    (``busy``) from a forward sweep that saved state for the reverse sweep until that reverse
+    sweep ran or the autograd graph was freed; a call finding it busy uses a fresh one.
+    """
+
+    def __init__(self, plan: DevicePlan) -> None:
+        self.plan = plan
+        self.bufs: dict[str, torch.Tensor] = {}
+        self.busy = False
+
+    def get(self, name: str, numel: int, dtype=None, zero: bool = False) -> torch.Tensor:
+        dt = dtype or self.plan.dtype
+        t = self.bufs.get(name)
+        if t is None or t.numel() < numel or t.dtype != dt:
+            t = torch.empty(max(1, numel), device=self.plan.device, dtype=dt)
+            self.bufs[name] = t
+        v = t[:numel]
+        if zero:
+            v.zero_()
+        return v
+
+    def state(self, prefix: str = "s") -> tuple[list[torch.Tensor], list[torch.Tensor]]:
+        P = self.plan
+        st = [self.get(f"{prefix}0", P.shot_numel, zero=True),
+              self.get(f"{prefix}1", P.shot_numel, zero=True)]
+        faces = [self.get(f"{prefix}f{i}", n, zero=True) for i, n in enumerate(P.face_numel * 2)]
+        return st, faces
+
+    def release(self) -> None:
+        self.busy = False
+
+
+class _Lease:
+    """Releases a held workspace when the saved reverse-sweep state is dropped."""
+
+    def __init__(self, ws: Workspace) -> None:
+        self._fin = weakref.finalize(self, ws.release)
+
+
 class Propagator:
     """Forward / reverse sweeps of one :class:`DevicePlan`."""
 
@@ -330,27 +385,40 @@ class Propagator:
         self.plan = plan
         self.grad_mode = grad_mode
         self.checkpoint_every = checkpoint_every
+        self.ws = Workspace(plan)
+
+    def _workspace(self, hold: bool) -> Workspace:
+        if self.ws.busy:
+            return Workspace(self.plan)
+        if hold:
+            self.ws.busy = True
+        return self.ws
 
     # ---- FWI: c -> traces ------------------------------------------------------------------
     def fwi_forward(self, c: torch.Tensor, need_grad: bool):
         P = self.plan
         h = P.host
-        m, ma = P.model(c, need_grad)
-        state, psi = P.new_state()
-        traces = P.zeros(h.n_trace_rows * h.n_t)
         N = h.n_steps
-        if not need_grad or N == 0:
+        hold = need_grad and N > 0
+        ws = self._workspace(hold)
+        m = ws.get("m", h.cells)
+        ma = ws.get("ma", h.cells) if need_grad else None
+        nat.check(nat.lib().nw_model(P.handle, c.data_ptr(), m.data_ptr(), nat.ptr(ma),
+                                     P.stream()), "nw_model")
+        state, psi = ws.state()
+        traces = ws.get("traces", h.n_trace_rows * h.n_t)
+        if not hold:
             P.forward(m, state, psi, 0, N, traces=traces)
-            return traces, None
+            return traces.clone(), None
         K = grad_block(P, N, self.grad_mode, self.checkpoint_every)
         nb = -(-N // K)
         last0 = (nb - 1) * K
-        ckpt = P.empty((nb - 1) * P.ckpt_slot) if nb > 1 else None
+        ckpt = ws.get("ckpt", (nb - 1) * P.ckpt_slot) if nb > 1 else None
         if last0 > 0:
             P.forward(m, state, psi, 0, last0, traces=traces, ckpt=ckpt, ckpt_every=K)
-        keep = P.empty((K + 1) * P.keep_slot)
+        keep = ws.get("keep", (K + 1) * P.keep_slot)
         P.forward(m, state, psi, last0, N, traces=traces, keep=keep)
-        return traces, (m, ma, ckpt, keep, K, nb)
+        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, _Lease(ws) if ws is self.ws else None)
 
     def fwi_backward(self, c: torch.Tensor, saved, residual: torch.Tensor) -> torch.Tensor:
         P = self.plan
@@ -359,13 +427,13 @@ class Propagator:
         grad_c = P.zeros(nf)
         if saved is None:
             return grad_c
-        m, ma, ckpt, keep, K, nb = saved
+        m, ma, ckpt, keep, K, nb, ws, _lease = saved
         N = h.n_steps
-        lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
-        psib = [P.zeros(n) for n in P.face_numel] * 2
-        psib = [t.clone() for t in psib]
-        grad_m = P.zeros(P.grad_slices() * h.cells, torch.float64)
-        state, psi = P.new_state()
+        res = ws.get("residual", residual.numel())
+        res.copy_(residual.reshape(-1))
+        lam, psib = ws.state("lam")
+        grad_m = ws.get("grad_m", P.grad_slices() * h.cells, torch.float64, zero=True)
+        state, psi = ws.state("s")
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
             if b < nb - 1:
@@ -376,36 +444,40 @@ class Propagator:
                     psi[a].copy_(f)
                 n_end = n1 if P.pml else max(n0, n1 - 1)
                 P.forward(m, state, psi, n0, n_end, keep=keep)
-            P.adjoint(ma, lam, psib, n0, n1, keep=keep, residual=residual,
+            P.adjoint(ma, lam, psib, n0, n1, keep=keep, residual=res,
                       inject_end=(b == nb - 1), grad=grad_m)
         nat.check(
             nat.lib().nw_model_grad(P.handle, c.data_ptr(), grad_m.data_ptr(), grad_c.data_ptr(),
                                     P.stream()),
             "nw_model_grad",
         )
+        ws.release()
         return grad_c
 
     # ---- PAT: p0 -> traces -----------------------------------------------------------------
     def ic_forward(self, m: torch.Tensor, p0: torch.Tensor) -> torch.Tensor:
         P = self.plan
         h = P.host
-        state, psi = P.new_state()
+        ws = self._workspace(False)
+        state, psi = ws.state()
         nat.check(
             nat.lib().nw_ic_state(P.handle, m.data_ptr(), p0.data_ptr(), state[0].data_ptr(),
                                   state[1].data_ptr(), P.stream()),
             "nw_ic_state",
         )
-        traces = P.zeros(h.n_trace_rows * h.n_t)
+        traces = ws.get("traces", h.n_trace_rows * h.n_t)
         P.forward(m, state, psi, 0, h.n_steps, traces=traces)
-        return traces
+        return traces.clone()
 
     def ic_backward(self, m: torch.Tensor, ma: torch.Tensor, residual: torch.Tensor) -> torch.Tensor:
         P = self.plan
         h = P.host
-        lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
-        psib = [P.zeros(n) for n in P.face_numel] + [P.zeros(n) for n in P.face_numel]
-        P.adjoint(ma, lam, psib, 0, h.n_steps, residual=residual, inject_end=True)
-        pbar = P.empty(P.shot_numel)
+        ws = self._workspace(False)
+        res = ws.get("residual", residual.numel())
+        res.copy_(residual.reshape(-1))
+        lam, psib = ws.state("lam")
+        P.adjoint(ma, lam, psib, 0, h.n_steps, residual=res, inject_end=True)
+        pbar = ws.get("pbar", P.shot_numel)
         nat.check(nat.lib().nw_prev_adjoint(P.handle, lam[1].data_ptr(), pbar.data_ptr(),
                                             P.stream()), "nw_prev_adjoint")
         nf = h.field_dims[0] * h.field_dims[1] * h.field_dims[2]

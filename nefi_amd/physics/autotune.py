@@ -81,7 +81,7 @@ def _time_kind(plan, kind: int, cand, steps: int) -> float:
     cells = P.host.cells
     m = P.zeros(cells)
     state, psi = P.new_state()
-    nat.profile_enable(True)
+    nat.profile_enable(True)  # (timed calls never go through graphs)
     if kind == 0:
         P.forward(m, state, psi, 0, steps)
         k = nat.K_STEP

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import logging
 import math
+import os
 from collections.abc import Callable, Sequence
 
 import torch
@@ -500,6 +501,9 @@ class _LeapfrogBase(Operator):
             prop = Propagator(DevicePlan(make_host(), device, dtype), self.grad_mode,
                               self.checkpoint_every)
             autotune.tune(prop.plan)
+            # each nw_forward / nw_adjoint call becomes one hipGraph launch after its first use
+            if os.environ.get("NEFI_AMD_GRAPH", "1") != "0":
+                prop.plan.set_option(nat.OPT_GRAPH, 1)
             self._cache[full] = prop
         prop.grad_mode = self.grad_mode  # instances switch it after construction
         prop.checkpoint_every = self.checkpoint_every
@@ -540,15 +544,16 @@ class _LeapfrogBase(Operator):
 
         obs = []
         n = 0
-        for r in rec:
-            if r > n:
-                P.forward(m, state, psi, n, r, first_step=first_step)
-                n = r
-            o = observe(snapshot(n), n)
-            if o is not None:
-                obs.append(o)
-        if n < n_steps:
-            P.forward(m, state, psi, n, n_steps, first_step=first_step)
+        with P.eager():
+            for r in rec:
+                if r > n:
+                    P.forward(m, state, psi, n, r, first_step=first_step)
+                    n = r
+                o = observe(snapshot(n), n)
+                if o is not None:
+                    obs.append(o)
+            if n < n_steps:
+                P.forward(m, state, psi, n, n_steps, first_step=first_step)
         return snapshot(n_steps), obs
 
     def extra_repr(self) -> str:
@@ -892,8 +897,9 @@ def time_reversal(
 
     state, psi = P.new_state()
     clamp(state[1], 0)
-    for n in range(op.n_steps):
-        P.forward(m, state, psi, n, n + 1)
-        clamp(state[1], n + 1)
+    with P.eager():
+        for n in range(op.n_steps):
+            P.forward(m, state, psi, n, n + 1)
+            clamp(state[1], n + 1)
     out = g.crop(P.host.wavefield_view(state[1], 1)[0])
     return out.to(device=tr.device, dtype=dtype).clone()

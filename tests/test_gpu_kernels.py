@@ -99,3 +99,31 @@ def test_fast_path_against_oracle_uneven_shots():
     ey, eg = rel(y, yo), rel(g, gs)
     print(f"uneven shots vs FP64 oracle: traces {ey:.2e}, gradient {eg:.2e}", flush=True)
     assert ey < 1e-5 and eg < 1e-4
+
+
+def test_hip_graph_replays_match_eager():
+    """Every call captured once, then replayed: each replay (with eager work and syncs between
+    them) must reproduce the eager traces and gradient exactly."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey("sponge", 4, n=(40, 88), shots=3, n_t=50)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, generator=torch.Generator(DEV).manual_seed(5))
+    x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
+    op({"c": x})
+    plan = next(iter(op._cache.values())).plan
+    plan.set_option(nat.OPT_GRAPH, 0)
+    y_e = op({"c": x})
+    (g_e,) = torch.autograd.grad((y_e * w).sum(), x)
+    plan.set_option(nat.OPT_GRAPH, 1)
+    counts = []
+    for rep in range(4):
+        y = op({"c": x})
+        (g,) = torch.autograd.grad((y * w).sum(), x)
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=DEV).sum().item()  # unrelated eager work between replays
+        counts.append(plan.option(nat.OPT_GRAPH_COUNT))
+        print(f"graph replay {rep}: graphs held {counts[-1]}", flush=True)
+        assert torch.equal(y, y_e) and torch.equal(g, g_e), rep
+    assert counts[-1] > 0
