@@ -90,6 +90,130 @@ def cpu_baseline(wl, budget_s=15.0):
     }
 
 
+def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
+    """The plain PyTorch autograd path of the reference on the GPU, for one sample of shots.
+
+    The reference's own op sequence for a sponge layer (nefi/physics/wave.py:529-552 inside
+    timestep.py:151-169): conv2d Laplacian with the (1,1,5,5) cross kernel, addcmul, a dense
+    source map times the host wavelet value, the damping products, multilinear gather at every
+    record step, and torch.utils.checkpoint over blocks of `block` steps. Returns dJ/dc for
+    J = sum(residual * traces).
+    """
+    import torch
+    import torch.nn.functional as F
+    from torch.utils.checkpoint import checkpoint
+
+    g = op.grid
+    dev, dt = c.device, op.dt
+    dtype = c.dtype
+    kern = torch.zeros(5, 5, dtype=torch.float64)
+    for ax, h in enumerate(g.spacing):
+        for off, co in ((-2, -1 / 12), (-1, 4 / 3), (0, -2.5), (1, 4 / 3), (2, -1 / 12)):
+            at = [2, 2]
+            at[ax] += off
+            kern[tuple(at)] += co / h**2
+    kern = kern.to(dev, dtype).view(1, 1, 5, 5)
+    sig = op.damping_profile()
+    half = 0.5 * sig * dt
+    a = (1.0 / (1.0 + half)).to(dev, dtype)
+    b = (1.0 - half).to(dev, dtype)
+    src = (g.source_density(torch.cat(op.shot_sources[:shots])) * dt**2).to(dev, dtype)
+    wav = [float(v) for v in op.wavelet()]
+    idx, w = op._rec
+    counts = op.receiver_counts[:shots]
+    rows = sum(counts)
+    ridx = idx[:rows].to(dev)
+    rw = w[:rows].to(dev, dtype)
+    shot_of = torch.repeat_interleave(torch.arange(shots), torch.tensor(counts)).to(dev)
+    flat_idx = shot_of[:, None] * (g.padded_shape[0] * g.padded_shape[1]) + ridx
+    record = set(op.record)
+
+    def sample(p):
+        return (p.reshape(-1)[flat_idx] * rw).sum(-1)
+
+    def step(p_prev, p, c2dt2, n):
+        lap = F.conv2d(p.reshape(-1, 1, *g.padded_shape), kern, padding=2).reshape(p.shape)
+        rhs = torch.addcmul(2.0 * p - b * p_prev, c2dt2, lap)
+        rhs = rhs + src * wav[n]
+        return p, a * rhs
+
+    x = c.detach().clone().requires_grad_(True)
+    c2dt2 = g.pad(x, "replicate") ** 2 * dt**2
+    p_prev = torch.zeros(shots, *g.padded_shape, device=dev, dtype=dtype)
+    p = torch.zeros_like(p_prev)
+    obs = [sample(p)]
+    for n0 in range(0, n_steps, block or n_steps):
+        n1 = min(n_steps, n0 + (block or n_steps))
+
+        def seg(pp, pc, m, n0=n0, n1=n1):
+            out = []
+            for n in range(n0, n1):
+                pp, pc = step(pp, pc, m, n)
+                if n + 1 in record:
+                    out.append(sample(pc))
+            return (pp, pc, *out)
+
+        # block=None: plain autograd, every step's tape kept (grad_mode="autograd")
+        res = seg(p_prev, p, c2dt2) if not block else checkpoint(seg, p_prev, p, c2dt2,
+                                                                 use_reentrant=False)
+        p_prev, p = res[0], res[1]
+        obs.extend(res[2:])
+    traces = torch.stack(obs, -1)
+    loss = (traces * residual[:rows, : traces.shape[-1]]).sum()
+    (grad,) = torch.autograd.grad(loss, x)
+    return grad
+
+
+def autograd_comparator(op, c, residual, shots=8, steps=200):
+    """Time the autograd path on `shots` shots x `steps` steps, with every step's tape kept
+    (grad_mode="autograd") and with checkpoint blocks of 200 (grad_mode="checkpoint"), and
+    scale linearly to the workload (a whole-workload autograd run needs terabytes)."""
+    import torch
+
+    scale = (op.n_steps / steps) * (len(op.shot_sources) / shots)
+    out = {}
+    for mode, block in (("autograd", None), ("checkpoint", 200)):
+        autograd_leapfrog(op, c, shots, 20, residual, block=block)  # warm up MIOpen, allocator
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        autograd_leapfrog(op, c, shots, steps, residual, block=block)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out[mode] = {
+            "ms_per_gradient_scaled": round(t * scale * 1e3, 1),
+            "sample": f"{shots} shots x {steps} steps measured {t:.2f} s, scaled x{scale:.0f} "
+                      f"to {len(op.shot_sources)} shots x {op.n_steps} steps",
+        }
+        torch.cuda.empty_cache()
+    return out
+
+
+def gradient_error_vs_fp64(wl, shots=8):
+    """Relative L2 error of the float32 gradient against the float64 one (the float64 HIP
+    path, equal to the reference's autograd to ~1e-15 in the parity tests) at the workload's
+    grid and time length, on the first `shots` shots."""
+    import torch
+
+    from nefi_amd.domain import Domain
+    from nefi_amd.physics.batched import ShotBatchWaveOperator
+
+    n = wl["n"]
+    dom = Domain((n, n), ((0.0, wl["L"]), (0.0, wl["L"])))
+    op = ShotBatchWaveOperator(dom, wl["shots"][:shots], grad_mode="checkpoint",
+                               checkpoint_every=200, **wl["kw"])
+    rows = int(sum(wl["counts"][:shots]))
+    gen = torch.Generator("cuda").manual_seed(7)
+    res64 = torch.randn(rows, op.n_t, device="cuda", dtype=torch.float64, generator=gen)
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        x = torch.tensor(wl["c"], dtype=dt, device="cuda", requires_grad=True)
+        y = op({"c": x})
+        (g,) = torch.autograd.grad((y * res64.to(dt)).sum(), x)
+        out[dt] = g.double()
+    err = float((out[torch.float32] - out[torch.float64]).norm() / out[torch.float64].norm())
+    return {"rel_l2": err, "sample": f"{shots} shots, full grid and {op.n_steps} steps"}
+
+
 def measured_traffic(kernel_key: str):
     """HBM bytes per launch of one kernel instance from the committed rocprofv3 counter runs
     (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration kernel)."""
@@ -115,6 +239,10 @@ def main() -> int:
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autograd", action="store_true",
+                    help="skip the plain-autograd comparator on the GPU")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the float32-vs-float64 gradient check")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 512 MiB read + 512 MiB write elementwise kernel "
                          "(reference point for HBM counters)")
@@ -229,6 +357,15 @@ def main() -> int:
     kernel_ms = {k: {"total_ms": round(v[0] / args.steps, 2), "launches": v[1] // args.steps}
                  for k, v in timing.items()}
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
+    comp = None
+    if not args.no_autograd:
+        comp = autograd_comparator(op, c, residual)
+        for entry in comp.values():
+            entry["speedup"] = round(entry["ms_per_gradient_scaled"] / ms_per_step, 1)
+            if graph_ms:
+                entry["speedup_hip_graph"] = round(entry["ms_per_gradient_scaled"] / graph_ms, 1)
+        print(f"[bench] autograd comparator: {comp}", file=sys.stderr, flush=True)
+    grad_err = None if args.no_accuracy else gradient_error_vs_fp64(wl)
     line = {
         "metric": "fwd+adjoint Gcell-updates/s (64-shot 2-D FWI gradient)",
         "value": round(value, 3),
@@ -258,6 +395,8 @@ def main() -> int:
         "kernels_per_gradient": kernel_ms,
         "tiles": tiles,
         "cpu_baseline": cpu,
+        "autograd_gpu": comp,
+        "gradient_error_vs_fp64": grad_err,
     }
     print(json.dumps(line), flush=True)
     return 0

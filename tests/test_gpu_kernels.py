@@ -127,3 +127,23 @@ def test_hip_graph_replays_match_eager():
         print(f"graph replay {rep}: graphs held {counts[-1]}", flush=True)
         assert torch.equal(y, y_e) and torch.equal(g, g_e), rep
     assert counts[-1] > 0
+
+
+def test_autograd_comparator_matches_hip_gradient():
+    """bench.py's plain-autograd path (the reference's aten op sequence) and the HIP reverse
+    sweep give the same gradient: they time the same computation."""
+    import bench
+
+    dom, geo, c, kw = _survey("sponge", 4, n=(40, 88), shots=3, n_t=50)
+    kw["checkpoint_every"] = 200
+    geo = [(s, r) for s, r in geo]
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    res = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float64,
+                      generator=torch.Generator(DEV).manual_seed(2))
+    x = torch.tensor(c, dtype=torch.float64, device=DEV, requires_grad=True)
+    (g_hip,) = torch.autograd.grad((op({"c": x}) * res).sum(), x)
+    g_ref = bench.autograd_leapfrog(op, x.detach(), 3, op.n_steps, res, block=17)
+    e = rel(g_ref.cpu().numpy(), g_hip.cpu().numpy())
+    print(f"autograd comparator vs HIP gradient (float64): {e:.2e}", flush=True)
+    assert e < 1e-10
