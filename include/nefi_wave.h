@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define NW_ABI_VERSION 2
+#define NW_ABI_VERSION 3
 
 enum nw_status {
   NW_OK = 0,
@@ -176,14 +176,21 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   so far are kept for when it is switched back on). Changing tiles or the other options
  *   drops the captured graphs. The whole time loop of a call is one graph launch.
  *   While kernel timing (nw_profile_enable) is on, calls run eagerly.
- * NW_OPT_GRAPH_COUNT (read only): graphs the plan currently holds. */
+ * NW_OPT_GRAPH_COUNT (read only): graphs the plan currently holds.
+ * NW_OPT_TIME_BLOCK: temporal blocking depth T of the fast forward step: 1 (default) one launch
+ *   per step; 2..4 advances up to T steps per launch, each LDS tile reading p^{n-1}, p^n and m
+ *   once (results bit-identical to T single steps). Used when the forward tiles have rows of
+ *   8, 16 or 32 and, for calls that record traces, a shot has at most 2048 receiver taps;
+ *   other calls step one by one. Setting T > 1 allocates two wavefields and a small tap
+ *   buffer held by the plan. */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,
   NW_OPT_LDS_TILES_STEP = 2,
   NW_OPT_LDS_TILES_ADJOINT = 3,
   NW_OPT_GRAPH = 4,
-  NW_OPT_GRAPH_COUNT = 5
+  NW_OPT_GRAPH_COUNT = 5,
+  NW_OPT_TIME_BLOCK = 6
 };
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
@@ -248,11 +255,19 @@ int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* value
  * by HIP events on its own stream, and calls run eagerly even in graph mode. nw_profile_read
  * waits for the recorded events and returns the summed duration (ms) and launch count of one
  * kernel kind since profiling was enabled: 0 = forward step, 1 = reverse step,
- * 2 = receiver sample, 3 = residual inject. Enabling clears earlier records. Process-wide; not
- * thread-safe. */
-enum nw_kernel_kind { NW_K_STEP = 0, NW_K_ADJOINT = 1, NW_K_SAMPLE = 2, NW_K_INJECT = 3 };
+ * 2 = receiver sample, 3 = residual inject, 4 = temporally blocked forward steps.
+ * nw_profile_units returns the time steps those launches advanced (for kind 2, the time
+ * samples recorded). Enabling clears earlier records. Process-wide; not thread-safe. */
+enum nw_kernel_kind {
+  NW_K_STEP = 0,
+  NW_K_ADJOINT = 1,
+  NW_K_SAMPLE = 2,
+  NW_K_INJECT = 3,
+  NW_K_STEPS = 4
+};
 int32_t nw_profile_enable(int32_t on);
 int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches);
+int32_t nw_profile_units(int32_t kind, int64_t* units);
 
 #ifdef __cplusplus
 }

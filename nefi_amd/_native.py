@@ -22,7 +22,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libnefi_wave.so"
 NW_NONE, NW_SPONGE, NW_CERJAN, NW_PML = 0, 1, 2, 3
 NW_F32, NW_F64 = 0, 1
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # every symbol include/nefi_wave.h declares
 EXPORTS = (
@@ -44,6 +44,7 @@ EXPORTS = (
     "nw_clamp",
     "nw_profile_enable",
     "nw_profile_read",
+    "nw_profile_units",
     "nw_plan_set_tiles",
     "nw_plan_get_tiles",
     "nw_grad_slices",
@@ -122,6 +123,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_clamp": (i32, [vp, vp, vp, vp, i32, vp]),
         "nw_profile_enable": (i32, [i32]),
         "nw_profile_read": (i32, [i32, POINTER(c_double), POINTER(c_int64)]),
+        "nw_profile_units": (i32, [i32, POINTER(c_int64)]),
         "nw_plan_set_tiles": (i32, [vp, i32, i32, i32]),
         "nw_plan_get_tiles": (i32, [vp, i32, POINTER(c_int32), POINTER(c_int32)]),
         "nw_grad_slices": (i64, [vp]),
@@ -161,13 +163,14 @@ def lib() -> ctypes.CDLL:
     return handle
 
 
-K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT = 0, 1, 2, 3
+K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT, K_STEPS = 0, 1, 2, 3, 4
 OPT_XCD_MAP = 0
 OPT_ROWS_PER_ITER = 1
 OPT_LDS_TILES_STEP = 2
 OPT_LDS_TILES_ADJOINT = 3
 OPT_GRAPH = 4
 OPT_GRAPH_COUNT = 5
+OPT_TIME_BLOCK = 6
 
 
 def profile_enable(on: bool) -> None:
@@ -179,6 +182,13 @@ def profile_read(kind: int) -> tuple[float, int]:
     ms, n = c_double(), c_int64()
     check(lib().nw_profile_read(kind, ctypes.byref(ms), ctypes.byref(n)), "nw_profile_read")
     return ms.value, n.value
+
+
+def profile_units(kind: int) -> int:
+    """Time steps advanced (samples recorded, for the sample kind) by those launches."""
+    u = c_int64()
+    check(lib().nw_profile_units(kind, ctypes.byref(u)), "nw_profile_units")
+    return u.value
 
 
 def check(rc: int, what: str) -> None:

@@ -4,7 +4,7 @@
 // layers of nefi (none, sponge, Cerjan taper, second-order PML), float or double. One thread
 // owns one padded cell of one shot and reads its stencil neighbours straight from global
 // memory (they are L1/L2 hits). The fast 2-D path lives in nefi_wave_2d.hip and is selected by
-// the driver for the shapes it supports.
+// the host loop for the shapes it supports.
 //
 // The step follows nefi/physics/wave.py:529-552 term by term:
 //   lap  = L p^n (+ PML: sum_a (G_a(psi'_a + psi_a)) / (2 h_a),  psi'_a = alpha psi_a + beta D_a p^n)
@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -38,6 +39,7 @@ int fail(int code, const std::string& msg) {
 // ------------------------------------------------------------------------------------------
 struct ProfRec {
   int kind;
+  int64_t units;  // time steps (or samples) the launch covers
   hipEvent_t start, stop;
 };
 
@@ -76,9 +78,9 @@ struct GraphEntry {
 };
 
 // brackets one launch; returns the record index or -1 when timing is off
-static int prof_begin(int kind, hipStream_t st) {
+static int prof_begin(int kind, hipStream_t st, int64_t units = 1) {
   if (!g_prof.on) return -1;
-  ProfRec r{kind, g_prof.get(), g_prof.get()};
+  ProfRec r{kind, units, g_prof.get(), g_prof.get()};
   if (!r.start || !r.stop) return -1;
   hipEventRecord(r.start, st);
   g_prof.recs.push_back(r);
@@ -433,6 +435,28 @@ __global__ void __launch_bounds__(256) k_clamp(int64_t cells, T* p, const int32_
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
+// Neutral-damping interval of axes 1 and 2 of a float plan with a sponge or Cerjan layer, from
+// the device profiles: the run of neutral cells around the middle of each axis.
+static int quiet_box(Geo<float>& g) {
+  for (int a = 1; a < 3; ++a) {
+    const int n = (int)g.D[a];
+    std::vector<float> v(n);
+    const hipError_t e = hipMemcpy(v.data(), g.damp[a], n * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+      return fail(NW_ERR_HIP, std::string("reading the damping profile: ") + hipGetErrorString(e));
+    const bool sponge = g.damp_kind == NW_SPONGE;
+    auto neutral = [&](int i) { return sponge ? v[i] == 0.f : v[i] >= 1.f; };
+    int lo = n / 2, hi = n / 2;
+    if (neutral(n / 2)) {
+      while (lo > 0 && neutral(lo - 1)) --lo;
+      while (hi < n && neutral(hi)) ++hi;
+    }
+    g.quiet[a][0] = lo;
+    g.quiet[a][1] = hi;
+  }
+  return NW_OK;
+}
+
 template <typename T>
 Geo<T> make_geo(const nw_desc& d) {
   Geo<T> g{};
@@ -445,6 +469,11 @@ Geo<T> make_geo(const nw_desc& d) {
   }
   g.P2 = d.pitch > 0 ? d.pitch : d.dims[2];
   g.cells = g.D[0] * g.D[1] * g.P2;
+  g.damp_kind = d.absorbing;
+  for (int a = 0; a < 3; ++a) {
+    g.quiet[a][0] = 0;
+    g.quiet[a][1] = (int)g.D[a];
+  }
   for (int a = 0; a < 3; ++a) {
     int64_t f = 1;
     for (int b = 0; b < 3; ++b) f *= g.D[b] + (b == a);
@@ -491,6 +520,12 @@ struct nw_plan {
   int xcd_map;  // NW_OPT_XCD_MAP
   int rows_per_iter;  // NW_OPT_ROWS_PER_ITER (forward step)
   int lds_tiles[2];   // NW_OPT_LDS_TILES_STEP / _ADJOINT
+  // temporal blocking of the fast forward step (NW_OPT_TIME_BLOCK): depth, the two extra
+  // wavefields a blocked sweep alternates with the caller's pair (its halo reads forbid
+  // writing in place), and the receiver-tap values of intermediate levels
+  int tblock;
+  float* tb_state[2];
+  float* tapbuf;
   // hipGraph capture of whole nw_forward / nw_adjoint calls (NW_OPT_GRAPH)
   int graph_mode;
   hipStream_t capture_stream;
@@ -541,9 +576,8 @@ template <>
 const Geo<double>& geo<double>(const nw_plan* p) { return p->gd; }
 
 template <typename T>
-static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t st) {
+static SampleIO<T> sample_io(const nw_plan* P, const T* p, T* traces, int j) {
   const nw_desc& d = P->d;
-  if (d.rec_max <= 0) return NW_OK;
   SampleIO<T> io{};
   io.p = p;
   io.traces = traces;
@@ -555,6 +589,14 @@ static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t s
   io.trace_first = d.trace_first;
   io.rec_idx = d.rec_idx;
   io.rec_w = (const T*)d.rec_w;
+  return io;
+}
+
+template <typename T>
+static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t st) {
+  const nw_desc& d = P->d;
+  if (d.rec_max <= 0) return NW_OK;
+  const SampleIO<T> io = sample_io<T>(P, p, traces, j);
   const int threads = d.rec_max * d.rec_taps;
   dim3 grid((threads + 255) / 256, d.n_shots);
   const int pid = prof_begin(NW_K_SAMPLE, st);
@@ -658,11 +700,150 @@ static void keep_psi(const nw_plan* P, T* slot, T** psi) {
   }
 }
 
+// Leave (prev, cur) of a finished sweep in the caller's pair (s0, s1): a pointer swap when
+// both already live there, copies for the ones that live in keep slots or plan buffers.
+template <typename T>
+static int hand_back(nw_plan* P, void** state, T* s0, T* s1, T* prev, T* cur, hipStream_t st) {
+  const bool prev_own = prev == s0 || prev == s1;
+  const bool cur_own = cur == s0 || cur == s1;
+  if (prev_own && cur_own) {
+    state[0] = prev;
+    state[1] = cur;
+  } else if (cur_own) {
+    T* other = cur == s0 ? s1 : s0;
+    NW_HIP(hipMemcpyAsync(other, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+    state[0] = other;
+    state[1] = cur;
+  } else if (prev_own) {
+    T* other = prev == s0 ? s1 : s0;
+    NW_HIP(hipMemcpyAsync(other, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+    state[0] = prev;
+    state[1] = other;
+  } else {
+    NW_HIP(hipMemcpyAsync(s0, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+    NW_HIP(hipMemcpyAsync(s1, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+    state[0] = s0;
+    state[1] = s1;
+  }
+  return NW_OK;
+}
+
+// The blocked sweep applies when the plan asks for it and the forward tiles have a blocked
+// kernel; a sweep recording traces also needs every shot's receiver taps in one scan.
+static bool use_blocked(const nw_plan* P, bool traces) {
+  const nw_desc& d = P->d;
+  return d.dtype == NW_F32 && P->tblock > 1 && P->rows[0] > 0 && P->tb_state[0] &&
+         fast2d_block_supported(P->rows[0], P->tblock) &&
+         (!traces || d.rec_max * d.rec_taps <= 256 * kTapChunks);
+}
+
+// Forward sweep in blocks of up to tblock steps (2-D float32 fast path). Blocks start at every
+// checkpoint step, so the kernel writes a checkpoint from the block's inputs. Outside keep
+// mode the block's last two levels go to the two of {s0, s1, tb_state} the block does not read.
+static int forward_blocked(nw_plan* P, const float* m, void** state, int n_begin, int n_end,
+                           float* traces, float* ckpt, int ckpt_every, float* keep,
+                           int keep_begin, int first_step, hipStream_t st) {
+  const nw_desc& d = P->d;
+  const int64_t keep_slot = nw_keep_slot_bytes(P);
+  const int64_t ck_slot = nw_ckpt_slot_bytes(P);
+  float* s0 = (float*)state[0];
+  float* s1 = (float*)state[1];
+  float* const bufs[4] = {s0, s1, P->tb_state[0], P->tb_state[1]};
+  auto kslot = [&](int n) {
+    return (float*)((char*)keep + (int64_t)(n - keep_begin) * keep_slot);
+  };
+  const bool keeping = keep != nullptr;
+  if (keeping && keep_begin == n_begin)
+    NW_HIP(hipMemcpyAsync(kslot(n_begin), s1, P->shot_bytes, hipMemcpyDeviceToDevice, st));
+  const bool taps = traces && d.rec_max > 0;
+  float* prev = s0;
+  float* cur = s1;
+  for (int n = n_begin; n < n_end;) {
+    if (traces && recorded(d, n)) NW_TRY(launch_sample<float>(P, cur, traces, n / d.substeps, st));
+    int L = std::min(P->tblock, n_end - n);
+    if (ckpt && ckpt_every > 0) L = std::min(L, (n / ckpt_every + 1) * ckpt_every - n);
+    BlockIO io{};
+    io.m = m;
+    io.p_prev = prev;
+    io.p = cur;
+    io.levels = L;
+    io.n0 = n;
+    io.first_step = first_step;
+    io.n_src = d.n_steps;
+    if (d.src_taps > 0) {
+      io.src_taps = d.src_taps;
+      io.src_idx = d.src_idx;
+      io.src_val = (const float*)d.src_val;
+      io.wavelet = (const float*)d.wavelet;
+    }
+    if (ckpt && ckpt_every > 0 && n % ckpt_every == 0) {
+      float* slot = (float*)((char*)ckpt + (int64_t)(n / ckpt_every) * ck_slot);
+      io.ck_prev = slot;
+      io.ck_cur = (float*)((char*)slot + P->shot_bytes);
+    }
+    if (keeping) {
+      for (int k = 1; k <= L; ++k) io.out[k] = kslot(n + k);
+    } else {
+      float* spare[2] = {nullptr, nullptr};
+      int ns = 0;
+      for (float* b : bufs)
+        if (b != prev && b != cur && ns < 2) spare[ns++] = b;
+      if (L >= 2) {
+        io.out[L - 1] = spare[0];
+        io.out[L] = spare[1];
+      } else {
+        io.out[1] = spare[0];
+      }
+    }
+    if (taps && L >= 2) {
+      io.tapbuf = P->tapbuf;
+      io.tap_stride = d.rec_max * d.rec_taps;
+      io.taps = d.rec_taps;
+      io.rec_first = d.rec_first;
+      io.rec_count = d.rec_count;
+      io.rec_idx = d.rec_idx;
+    }
+    const int pid = prof_begin(NW_K_STEPS, st, L);
+    fast2d_steps(P->gf, d.order, d.absorbing, io, d.n_shots, P->rows[0], P->tblock, P->spw[0],
+                 P->xcd_map, st);
+    prof_end(pid, st);
+    NW_TRY(check_launch("k4_steps"));
+    if (io.tapbuf) {
+      LevelRecords lj;
+      int nrec = 0;
+      for (int k = 0; k < kMaxDepth; ++k) lj.j[k] = -1;
+      for (int k = 1; k < L; ++k)
+        if (recorded(d, n + k)) {
+          lj.j[k - 1] = (n + k) / d.substeps;
+          ++nrec;
+        }
+      if (nrec > 0) {
+        const int pid2 = prof_begin(NW_K_SAMPLE, st, nrec);
+        fast2d_sample_levels(P->gf, sample_io<float>(P, nullptr, traces, 0), P->tapbuf,
+                             io.tap_stride, d.rec_max, d.n_shots, L, lj, st);
+        prof_end(pid2, st);
+        NW_TRY(check_launch("k_sample_levels"));
+      }
+    }
+    prev = L >= 2 ? io.out[L - 1] : cur;
+    cur = io.out[L];
+    n += L;
+  }
+  if (traces && recorded(d, n_end)) NW_TRY(launch_sample<float>(P, cur, traces, n_end / d.substeps, st));
+  if (n_end > n_begin) NW_TRY(hand_back<float>(P, state, s0, s1, prev, cur, st));
+  return NW_OK;
+}
+
 template <typename T>
 static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_begin, int n_end,
                         T* traces, T* ckpt, int ckpt_every, T* keep, int keep_begin,
                         int first_step, hipStream_t st) {
   const nw_desc& d = P->d;
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_blocked(P, traces != nullptr))
+      return forward_blocked(P, m, state, n_begin, n_end, traces, ckpt, ckpt_every, keep,
+                             keep_begin, first_step, st);
+  }
   const bool pml = d.absorbing == NW_PML;
   const int64_t keep_slot = nw_keep_slot_bytes(P);
   const int64_t ck_slot = nw_ckpt_slot_bytes(P);
@@ -746,30 +927,7 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
   if (traces && recorded(d, n_end)) NW_TRY(launch_sample<T>(P, cur, traces, n_end / d.substeps, st));
   // hand the final state back in the caller's buffers
   if (n_end > n_begin) {
-    const bool prev_own = prev == s0 || prev == s1;
-    const bool cur_own = cur == s0 || cur == s1;
-    if (prev_own && cur_own) {
-      state[0] = prev;
-      state[1] = cur;
-    } else {
-      // at least one lives in a keep slot: copy into the caller's buffers
-      if (cur_own) {
-        T* other = cur == s0 ? s1 : s0;
-        NW_HIP(hipMemcpyAsync(other, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
-        state[0] = other;
-        state[1] = cur;
-      } else if (prev_own) {
-        T* other = prev == s0 ? s1 : s0;
-        NW_HIP(hipMemcpyAsync(other, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
-        state[0] = prev;
-        state[1] = other;
-      } else {
-        NW_HIP(hipMemcpyAsync(s0, prev, P->shot_bytes, hipMemcpyDeviceToDevice, st));
-        NW_HIP(hipMemcpyAsync(s1, cur, P->shot_bytes, hipMemcpyDeviceToDevice, st));
-        state[0] = s0;
-        state[1] = s1;
-      }
-    }
+    NW_TRY(hand_back<T>(P, state, s0, s1, prev, cur, st));
     if (pml) {
       int k = 0;
       for (int a = 0; a < 3; ++a) {
@@ -955,6 +1113,15 @@ int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches) {
   return NW_OK;
 }
 
+int32_t nw_profile_units(int32_t kind, int64_t* units) {
+  if (!units) return fail(NW_ERR_ARG, "nw_profile_units: null argument");
+  int64_t u = 0;
+  for (auto& r : g_prof.recs)
+    if (r.kind == kind) u += r.units;
+  *units = u;
+  return NW_OK;
+}
+
 const char* nw_last_error(void) { return nw::g_err.c_str(); }
 
 int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
@@ -994,6 +1161,13 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->gf = make_geo<float>(d);
   p->gd = make_geo<double>(d);
   const bool fast = fast2d_supported(p->gf, d.absorbing, d.dtype);
+  if (fast && d.absorbing != NW_NONE) {
+    const int rc = quiet_box(p->gf);
+    if (rc != NW_OK) {
+      delete p;
+      return rc;
+    }
+  }
   for (int k = 0; k < 2; ++k) {
     p->rows[k] = fast ? 16 : 0;
     p->spw[k] = 1;
@@ -1001,6 +1175,9 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->xcd_map = 1;
   p->rows_per_iter = 1;
   p->lds_tiles[0] = p->lds_tiles[1] = 0;
+  p->tblock = 1;
+  p->tb_state[0] = p->tb_state[1] = nullptr;
+  p->tapbuf = nullptr;
   p->graph_mode = 0;
   p->capture_stream = nullptr;
   p->graph_serial = 0;
@@ -1019,6 +1196,8 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   if (!plan) return NW_OK;
   drop_graphs(plan);
   if (plan->capture_stream) hipStreamDestroy(plan->capture_stream);
+  for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf})
+    if (b) hipFree(b);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)
     if (nw::g_plans[i] == plan) {
       nw::g_plans.erase(nw::g_plans.begin() + i);
@@ -1085,6 +1264,21 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     P->lds_tiles[k] = value != 0;
     return NW_OK;
   }
+  if (option == NW_OPT_TIME_BLOCK) {
+    if (value < 1 || value > kMaxDepth)
+      return fail(NW_ERR_ARG, "time block depth must be 1.." + std::to_string(kMaxDepth));
+    if (value > 1 && !P->tb_state[0]) {
+      if (P->d.dtype != NW_F32 || !fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
+        return fail(NW_ERR_UNSUPPORTED, "time blocking needs the fast 2-D float32 kernels");
+      const int64_t taps = (int64_t)(kMaxDepth - 1) * P->d.n_shots *
+                           std::max(1, P->d.rec_max * P->d.rec_taps);
+      for (int k = 0; k < 2; ++k)
+        NW_HIP(hipMalloc((void**)&P->tb_state[k], (size_t)P->shot_bytes));
+      NW_HIP(hipMalloc((void**)&P->tapbuf, (size_t)taps * sizeof(float)));
+    }
+    P->tblock = value;
+    return NW_OK;
+  }
   return fail(NW_ERR_ARG, "nw_plan_set_option: unknown option");
 }
 
@@ -1108,6 +1302,10 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_LDS_TILES_STEP || option == NW_OPT_LDS_TILES_ADJOINT) {
     *value = P->lds_tiles[option == NW_OPT_LDS_TILES_STEP ? 0 : 1];
+    return NW_OK;
+  }
+  if (option == NW_OPT_TIME_BLOCK) {
+    *value = P->tblock;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

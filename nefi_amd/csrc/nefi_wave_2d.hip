@@ -9,7 +9,10 @@
 // workgroup advances one after the other, so the shared rows of m (and, in the reverse sweep,
 // the float64 gradient tile of the group) are re-read from cache rather than from HBM.
 //
-// Arithmetic follows the generic kernels (nefi_wave.hip) operation by operation.
+// Arithmetic: the same terms as the generic kernels (nefi_wave.hip). Implicit contraction of
+// multiplies and adds is off in this file and every fused multiply-add is written out, so the
+// rounding of each kernel is fixed by its source: the tile variants below (register march, LDS
+// tiles, temporal blocks, damped and layer-free tiles) give bit-identical cell values.
 
 #include <hip/hip_runtime.h>
 
@@ -17,6 +20,8 @@
 
 #include "nefi_wave.h"
 #include "nefi_wave_internal.h"
+
+#pragma clang fp contract(off)
 
 namespace nw {
 
@@ -45,13 +50,10 @@ template <int R>
 __device__ __forceinline__ float stencil1(const Geo<float>& g, float c, float u1, float d1,
                                           float u2, float d2, float l1, float r1, float l2,
                                           float r2) {
-  float s = g.cc * c;
-  float ay = g.c1[1] * (u1 + d1);
-  if (R == 2) ay += g.c2[1] * (u2 + d2);
-  s += ay;
-  float ax = g.c1[2] * (l1 + r1);
-  if (R == 2) ax += g.c2[2] * (l2 + r2);
-  s += ax;
+  float s = __builtin_fmaf(g.c1[1], u1 + d1, g.cc * c);
+  if (R == 2) s = __builtin_fmaf(g.c2[1], u2 + d2, s);
+  s = __builtin_fmaf(g.c1[2], l1 + r1, s);
+  if (R == 2) s = __builtin_fmaf(g.c2[2], l2 + r2, s);
   return s;
 }
 
@@ -167,7 +169,7 @@ template <int ABS>
 __device__ __forceinline__ float rhs1(float pc, float prev_raw, float m, float lap, float b,
                                       float tap, int taper_prev) {
   const float prev = (ABS == NW_CERJAN && taper_prev) ? tap * prev_raw : prev_raw;
-  return (2.f * pc - b * prev) + m * lap;
+  return __builtin_fmaf(m, lap, __builtin_fmaf(-b, prev, 2.f * pc));
 }
 
 // adjoint of one cell: lambda^n from rbar = a lambda^{n+1}, the next step's lambda^{n+2} term
@@ -178,11 +180,27 @@ __device__ __forceinline__ float adj1(float lc, float l2c, float lapq, float a, 
   rbar = a * lc;
   const float tp1 = (ABS == NW_CERJAN && taper_next) ? tap : 1.f;
   const float pbar_next = -(b * tp1) * (a * l2c);
-  return pbar_next + 2.f * rbar + lapq;
+  return __builtin_fmaf(2.f, rbar, pbar_next) + lapq;
 }
 
 __device__ __forceinline__ float4 mul4(const float4& a, const float4& b) {
   return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
+}
+
+// Damping of a lane's four columns in a row whose row term is zero, computed once per lane:
+// with hy = 0 the sponge's half = hy + hx is hx exactly, so a = 1 / (1 + hx) and b = 1 - hx
+// are the per-cell values bit for bit, and the division leaves the per-row work.
+template <int ABS>
+__device__ __forceinline__ Damp4<ABS> damp4_col(const float4& hx) {
+  return damp4<ABS>(0.f, hx);
+}
+
+// damping of one row: the precomputed column values when the row term vanishes (sponge)
+template <int ABS>
+__device__ __forceinline__ Damp4<ABS> damp4_row(float hy, const float4& hx,
+                                                const Damp4<ABS>& col) {
+  if (ABS == NW_SPONGE && hy == 0.f) return col;
+  return damp4<ABS>(hy, hx);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -349,10 +367,10 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
         st4(io.lam0 + base + ro, o);
         if (gp) {
           const float4 lapp = stencil4<R>(g, pw, pnb.x, pnb.y, pnb.z, pnb.w);
-          g0.x += (double)rb.x * (double)lapp.x;
-          g0.y += (double)rb.y * (double)lapp.y;
-          g1.x += (double)rb.z * (double)lapp.z;
-          g1.y += (double)rb.w * (double)lapp.w;
+          g0.x = __builtin_fma((double)rb.x, (double)lapp.x, g0.x);
+          g0.y = __builtin_fma((double)rb.y, (double)lapp.y, g0.y);
+          g1.x = __builtin_fma((double)rb.z, (double)lapp.z, g1.x);
+          g1.y = __builtin_fma((double)rb.w, (double)lapp.w, g1.y);
           gp[0] = g0;
           gp[1] = g1;
         }
@@ -435,6 +453,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
   const int r0 = wave * S::my_rows;  // first tile row of this wavefront
   const bool act = x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<ABS> dcol = damp4_col<ABS>(hx);
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
   // model rows of this wavefront: shot-independent, loaded once for the group
   float4 mm[S::my_rows];
@@ -470,7 +489,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
         const int ro = y * P2 + x;
         const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
         const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+        const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
         const float4 pc = w[R];
         const int tpv = io.taper_prev;
         float4 o = make_float4(rhs1<ABS>(pc.x, pp[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
@@ -500,23 +519,39 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
   }
 }
 
-template <int R, int ABS, int TY>
-__global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io, int shots,
-                                                  int spw, int xcd_map) {
+// One tile of k3_adjoint. Q tiles (tile, halo and margins inside the grid, output cells in the
+// box of neutral damping) skip every bounds check and step with a = b = 1, as k4_tile does.
+// The tile of m a_eff is shot-independent: it stays in registers for the whole shot group.
+template <int R, int ABS0, int TY, bool Q>
+__device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<float>& io,
+                                            int shots, int spw, const Tile& tl, float* lq,
+                                            float* lp) {
+  constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = TileShape<TY, R>;
-  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
-  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
-  if (!tl.valid) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;
-  const bool act = x < D2;
+  const bool act = Q || x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<ABS> dcol = damp4_col<ABS>(hx);
   const bool imaging = io.p_n != nullptr && io.grad != nullptr;
   const int tpn = io.taper_next;
+  // tile loads: every element inside the grid for Q tiles
+  auto load_tile = [&](const float* src, float4* v) {
+#pragma unroll
+    for (int k = 0; k < S::per_thread; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+      const int gy = y0 - R + row, gx = x0 - 4 + 4 * c4;
+      const bool in = 256 * (k + 1) <= S::n4 || i < S::n4;
+      const bool ok = in && (Q || (gy >= 0 && gy < D1 && gx >= 0 && gx < D2));
+      v[k] = ok ? ld4(src + (gy * P2 + gx)) : f4(0.f);
+    }
+  };
+  float4 vm[S::per_thread];
+  load_tile(io.ma, vm);
   // float64 imaging sums of this wavefront's cells over the shot group
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
@@ -529,9 +564,8 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
     const int64_t base = (int64_t)s * g.cells;
     const float* L1 = io.lam1 + base;
     {
-      float4 vl[S::per_thread], vm[S::per_thread];
-      tile_load<TY, R>(L1, y0, x0, D1, D2, P2, vl);
-      tile_load<TY, R>(io.ma, y0, x0, D1, D2, P2, vm);
+      float4 vl[S::per_thread];
+      load_tile(L1, vl);
       __syncthreads();  // the previous shot's reads of lq / lp are done
 #pragma unroll
       for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
@@ -539,14 +573,14 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
     }
     if (imaging) {
       float4 vp[S::per_thread];
-      tile_load<TY, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
+      load_tile(io.p_n + base, vp);
       tile_store<TY, R>(lp, vp);
     }
     float4 lc[S::my_rows], l2c[S::my_rows];
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
-      const bool ok = act && y < D1;
+      const bool ok = Q || (act && y < D1);
       lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // just loaded into the tile: cache hit
       l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
     }
@@ -562,11 +596,11 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
       const int y = y0 + r0 + k;
       qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
       pw[2 * R] = imaging ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
-      if (act && y < D1) {
+      if (Q || (act && y < D1)) {
         const float4 qn = lnb(lq, r0 + k + R, lane);
         const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
         const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const Damp4<ABS> dm = damp4<ABS>(hy, hx);
+        const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
         float4 rb;
         const float4 o = make_float4(
             adj1<ABS>(lc[k].x, l2c[k].x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
@@ -577,10 +611,10 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
         if (imaging) {
           const float4 pn = lnb(lp, r0 + k + R, lane);
           const float4 lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
-          ga[k].x += (double)rb.x * (double)lapp.x;
-          ga[k].y += (double)rb.y * (double)lapp.y;
-          gb[k].x += (double)rb.z * (double)lapp.z;
-          gb[k].y += (double)rb.w * (double)lapp.w;
+          ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
+          ga[k].y = __builtin_fma((double)rb.y, (double)lapp.y, ga[k].y);
+          gb[k].x = __builtin_fma((double)rb.z, (double)lapp.z, gb[k].x);
+          gb[k].y = __builtin_fma((double)rb.w, (double)lapp.w, gb[k].y);
         }
       }
 #pragma unroll
@@ -595,7 +629,7 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
-      if (act && y < D1) {
+      if (Q || (act && y < D1)) {
         double2* gp = reinterpret_cast<double2*>(G + (y * P2 + x));
         double2 a = gp[0], b = gp[1];
         a.x += ga[k].x;
@@ -607,6 +641,270 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
       }
     }
   }
+}
+
+template <int R, int ABS, int TY>
+__global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io, int shots,
+                                                  int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
+  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  const bool quiet = y0 - R >= 0 && y0 + TY + R <= D1 && x0 >= 4 && x0 + 260 <= D2 &&
+                     y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
+                     x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
+  if (quiet)
+    k3_adj_tile<R, ABS, TY, true>(g, io, shots, spw, tl, lq, lp);
+  else
+    k3_adj_tile<R, ABS, TY, false>(g, io, shots, spw, tl, lq, lp);
+}
+
+// ------------------------------------------------------------------------------------------
+// temporally blocked forward steps
+// ------------------------------------------------------------------------------------------
+// One workgroup advances one tile of TY output rows x (256 - 2 HX) output columns by up to T
+// steps, reading p^{n-1}, p^n and m once and writing only the levels the caller keeps. The
+// loaded region spans rows y0 - HY .. y0 + TY + HY - 1 (HY = R T) and the 256 columns
+// x0 - HX .. x0 - HX + 255 (HX = R T rounded up to a float4). Every level is computed on the
+// whole region from the previous level in LDS; level k is exact on the region shrunk by k R
+// on each side, so after T levels the output tile is exact, and the values in the shrinking
+// rim never reach it. Cells outside the grid stay zero, which is what the unblocked stencil
+// reads there. Per cell the arithmetic is the unblocked step's, operation by operation, so
+// the levels are bit-identical to T launches of k3_step.
+//
+// p^{n-1} of a lane's rows is pointwise and stays in registers (so do p^n and m); only the
+// level the stencil reads goes through LDS, one buffer with a barrier between the reads of a
+// level and the writes of the next. Receiver taps that fall in the output tile are read out of
+// LDS at every intermediate level into a tap buffer, which a small kernel turns into trace
+// samples with k_sample's tap order; the last level is sampled from memory as usual.
+constexpr int kBlockWaves = 8;                  // wavefronts of one blocked workgroup
+constexpr int kBlockThreads = 64 * kBlockWaves;
+constexpr int kBlockTapChunks = kTapChunks * 256 / kBlockThreads;
+
+template <int TY, int R, int T>
+struct BlockShape {
+  static constexpr int hy = R * T;
+  static constexpr int hx = (R * T + 3) / 4 * 4;
+  static constexpr int out_w = 256 - 2 * hx;
+  static constexpr int rows = TY + 2 * hy;                         // region rows
+  static constexpr int mr = (rows + kBlockWaves - 1) / kBlockWaves;  // rows of one wavefront
+  static constexpr int crow = kBlockWaves * mr;                    // computed rows
+  static constexpr int lrows = crow + 2 * R;  // LDS rows: R zero rows above and below
+  static constexpr int n4 = crow * (kLW / 4);  // float4 loads of one region
+  static constexpr int per_thread = (n4 + kBlockThreads - 1) / kBlockThreads;
+};
+
+// One tile of k4_steps. Q ("quiet") tiles lie inside the grid, margins included, and inside
+// the box of neutral damping: every bounds check is dead there and a = b = 1 (taper 1), so
+// they step with the arithmetic of the layer-free step, which is bit-identical to the
+// damped arithmetic at those coefficients (products by 1 and 2 p - 1 p' are exact).
+template <int R, int ABS0, int TY, int T, bool Q>
+__device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, int shots,
+                                        int spw, const Tile& tl, float* lp) {
+  constexpr int ABS = Q ? NW_NONE : ABS0;
+  using S = BlockShape<TY, R, T>;
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ry0 = tl.ry * TY - S::hy;        // grid row of region row 0
+  const int rx0 = tl.cx * S::out_w - S::hx;  // grid column of region column 0
+  const int x = rx0 + 4 * lane;
+  const bool inx = Q || (x >= 0 && x < D2);
+  const bool outx = inx && lane >= S::hx / 4 && lane < 64 - S::hx / 4;
+  const int c0 = wave * S::mr;  // first region row of this wavefront
+  const float4 hx = (ABS != NW_NONE && inx) ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<ABS> dcol = damp4_col<ABS>(hx);
+  float4 mm[S::mr];
+#pragma unroll
+  for (int k = 0; k < S::mr; ++k) {
+    const int y = ry0 + c0 + k;
+    mm[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+  }
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
+    const int64_t base = (int64_t)s * g.cells;
+    float4 pv[S::mr], cur[S::mr];
+    {
+      float4 v[S::per_thread];
+      const float* P = io.p + base;
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) {
+        const int i = threadIdx.x + kBlockThreads * k;
+        const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+        const int gy = ry0 + row, gx = rx0 - 4 + 4 * c4;
+        const bool in = (kBlockThreads * (k + 1) <= S::n4 || i < S::n4);
+        const bool ok = in && (Q || (gy >= 0 && gy < D1 && gx >= 0 && gx < D2));
+        v[k] = ok ? ld4(P + (gy * P2 + gx)) : f4(0.f);
+      }
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const int y = ry0 + c0 + k;
+        pv[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.p_prev + base + (y * P2 + x))
+                                                 : f4(0.f);
+      }
+      __syncthreads();  // the previous shot's reads of lp are done
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) {
+        const int i = threadIdx.x + kBlockThreads * k;
+        if (kBlockThreads * (k + 1) <= S::n4 || i < S::n4)
+          *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
+      }
+    }
+    // does a source tap of this shot fall in this wavefront's rows of the region? (uniform)
+    bool src_rows = false;
+    for (int t = 0; t < io.src_taps; ++t) {
+      const int idx = io.src_idx[s * io.src_taps + t];
+      const int sy = idx / P2 - ry0 - c0;
+      src_rows |= sy >= 0 && sy < S::mr;
+    }
+    // receiver taps of this shot inside the output tile: LDS offset, or -1
+    int toff[kBlockTapChunks];
+    const int n_tap = io.tapbuf ? io.rec_count[s] * io.taps : 0;
+    const int tap0 = io.tapbuf ? io.rec_first[s] * io.taps : 0;
+#pragma unroll
+    for (int c = 0; c < kBlockTapChunks; ++c) {
+      const int e = threadIdx.x + kBlockThreads * c;
+      toff[c] = -1;
+      if (e < n_tap) {
+        const int idx = io.rec_idx[tap0 + e];
+        const int ty = idx / P2, tx = idx - ty * P2;
+        if (ty >= ry0 + S::hy && ty < ry0 + S::hy + TY && tx >= rx0 + S::hx &&
+            tx < rx0 + 256 - S::hx)
+          toff[c] = (R + ty - ry0) * kLW + 4 + (tx - rx0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+    if (io.ck_prev) {
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const int c = c0 + k, y = ry0 + c;
+        if (outx && c >= S::hy && c < S::hy + TY && (Q || y < D1)) {
+          st4(io.ck_prev + base + (y * P2 + x), pv[k]);
+          st4(io.ck_cur + base + (y * P2 + x), cur[k]);
+        }
+      }
+    }
+    for (int lvl = 1; lvl <= io.levels; ++lvl) {
+      const int n = io.n0 + lvl - 1;
+      const bool src = src_rows && n < io.n_src;
+      const float w_n = src ? io.wavelet[n] : 0.f;
+      const int tpv = n > io.first_step;
+      float4 w[2 * R + 1];
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) w[j] = lrow(lp, c0 + j, lane);
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const int y = ry0 + c0 + k;
+        w[2 * R] = lrow(lp, c0 + k + 2 * R, lane);
+        float4 o = f4(0.f);
+        if (Q || (y >= 0 && y < D1)) {  // uniform; columns outside the grid are zeroed below
+          const float4 nb = lnb(lp, c0 + k + R, lane);
+          const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+          const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+          const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
+          const float4 pc = cur[k];
+          o = make_float4(rhs1<ABS>(pc.x, pv[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
+                          rhs1<ABS>(pc.y, pv[k].y, mm[k].y, lap.y, dm.b.y, dm.tap.y, tpv),
+                          rhs1<ABS>(pc.z, pv[k].z, mm[k].z, lap.z, dm.b.z, dm.tap.z, tpv),
+                          rhs1<ABS>(pc.w, pv[k].w, mm[k].w, lap.w, dm.b.w, dm.tap.w, tpv));
+          if (src) {
+            const int ro = y * P2 + x;
+            for (int t = 0; t < io.src_taps; ++t) {
+              const int d = io.src_idx[s * io.src_taps + t] - ro;
+              if (d >= 0 && d < 4) {
+                const float sv = io.src_val[s * io.src_taps + t] * w_n;
+                o.x += d == 0 ? sv : 0.f;
+                o.y += d == 1 ? sv : 0.f;
+                o.z += d == 2 ? sv : 0.f;
+                o.w += d == 3 ? sv : 0.f;
+              }
+            }
+          }
+          if (ABS != NW_NONE) o = mul4(dm.a, o);
+          if (!Q && !inx) o = f4(0.f);
+        }
+        pv[k] = o;  // p^{n-1} is consumed: the slot takes the new level
+#pragma unroll
+        for (int j = 0; j < 2 * R; ++j) w[j] = w[j + 1];
+      }
+      if (lvl < io.levels) {
+        __syncthreads();  // every read of the previous level is done
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k)
+          *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = pv[k];
+        __syncthreads();
+        if (io.tapbuf) {
+          float* tb = io.tapbuf + ((int64_t)(lvl - 1) * shots + s) * io.tap_stride;
+#pragma unroll
+          for (int c = 0; c < kBlockTapChunks; ++c)
+            if (toff[c] >= 0) tb[threadIdx.x + kBlockThreads * c] = lp[toff[c]];
+        }
+      }
+      float* out = io.out[lvl];
+      if (out) {
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k) {
+          const int c = c0 + k, y = ry0 + c;
+          if (outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
+            st4(out + base + (y * P2 + x), pv[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const float4 t = cur[k];
+        cur[k] = pv[k];
+        pv[k] = t;
+      }
+    }
+  }
+}
+
+template <int R, int ABS, int TY, int T>
+__global__ void __launch_bounds__(kBlockThreads) k4_steps(Geo<float> g, BlockIO io, int shots,
+                                                          int spw, int xcd_map) {
+  using S = BlockShape<TY, R, T>;
+  __shared__ __align__(16) float lp[S::lrows * kLW];
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
+  const Tile tl = tile_of(div_up(D2, S::out_w), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  for (int i = threadIdx.x; i < R * kLW; i += kBlockThreads) {
+    lp[i] = 0.f;
+    lp[(S::lrows - R) * kLW + i] = 0.f;
+  }
+  const int ry0 = tl.ry * TY - S::hy, rx0 = tl.cx * S::out_w - S::hx;
+  const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
+                     ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&
+                     rx0 >= g.quiet[2][0] && rx0 + 256 <= g.quiet[2][1];
+  if (quiet)
+    k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lp);
+  else
+    k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lp);
+}
+
+// trace samples of the intermediate levels of a blocked launch, from its tap buffer: the
+// products and the tap sum of k_sample, in its order
+__global__ void __launch_bounds__(256) k_sample_levels(Geo<float> g, SampleIO<float> io,
+                                                       const float* tapbuf, int tap_stride,
+                                                       int shots, LevelRecords lj) {
+  (void)g;
+  const int s = blockIdx.y, lvl = blockIdx.z;
+  const int j = lj.j[lvl];
+  if (j < 0) return;  // uniform over the workgroup
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int taps = io.taps;
+  const int r = t / taps, k = t % taps;
+  const int count = io.rec_count[s];
+  float v = 0.f;
+  if (r < count) {
+    const int64_t row = (int64_t)io.rec_first[s] + r;
+    v = io.rec_w[row * taps + k] * tapbuf[((int64_t)lvl * shots + s) * tap_stride + t];
+  }
+  for (int off = 1; off < taps; off <<= 1) v += __shfl_xor(v, off, 64);
+  if (r < count && k == 0) io.traces[((int64_t)io.trace_first[s] + r) * io.n_t + j] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -705,6 +1003,64 @@ void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<f
     case NW_CERJAN * 2 + 0: adj_ty<1, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
     default: adj_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
   }
+}
+
+bool fast2d_block_supported(int ty, int depth) {
+  return (ty == 8 || ty == 16 || ty == 32) && depth >= 2 && depth <= kMaxDepth;
+}
+
+template <int R, int ABS, int TY, int T>
+static void launch_steps_t(const Geo<float>& g, const BlockIO& io, int shots, int spw, int xm,
+                           hipStream_t st) {
+  using S = BlockShape<TY, R, T>;
+  const int tiles =
+      div_up((int)g.D[2], S::out_w) * div_up((int)g.D[1], TY) * div_up(shots, spw);
+  hipLaunchKernelGGL((k4_steps<R, ABS, TY, T>), dim3((unsigned)(kXcds * div_up(tiles, kXcds))),
+                     dim3(kBlockThreads), 0, st, g, io, shots, spw, xm & 1);
+}
+
+template <int R, int ABS, int TY>
+static void steps_depth(const Geo<float>& g, const BlockIO& io, int shots, int depth, int spw,
+                        int xm, hipStream_t st) {
+  switch (depth) {
+    case 2: launch_steps_t<R, ABS, TY, 2>(g, io, shots, spw, xm, st); break;
+    case 3: launch_steps_t<R, ABS, TY, 3>(g, io, shots, spw, xm, st); break;
+    default: launch_steps_t<R, ABS, TY, 4>(g, io, shots, spw, xm, st); break;
+  }
+}
+
+template <int R, int ABS>
+static void steps_ty(const Geo<float>& g, const BlockIO& io, int shots, int ty, int depth,
+                     int spw, int xm, hipStream_t st) {
+  switch (ty) {
+    case 8: steps_depth<R, ABS, 8>(g, io, shots, depth, spw, xm, st); break;
+    case 32: steps_depth<R, ABS, 32>(g, io, shots, depth, spw, xm, st); break;
+    default: steps_depth<R, ABS, 16>(g, io, shots, depth, spw, xm, st); break;
+  }
+}
+
+void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& io, int shots,
+                  int ty, int depth, int spw, int xm, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int key = absorbing * 2 + (order == 4 ? 1 : 0);
+  switch (key) {
+    case NW_NONE * 2 + 0: steps_ty<1, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_NONE * 2 + 1: steps_ty<2, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_SPONGE * 2 + 0: steps_ty<1, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_SPONGE * 2 + 1: steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_CERJAN * 2 + 0: steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    default: steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+  }
+}
+
+void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const float* tapbuf,
+                          int tap_stride, int rec_max, int shots, int levels,
+                          const LevelRecords& lj, void* stream) {
+  if (levels < 2 || rec_max <= 0) return;
+  const dim3 grid((unsigned)div_up(rec_max * io.taps, 256), (unsigned)shots,
+                  (unsigned)(levels - 1));
+  hipLaunchKernelGGL(k_sample_levels, grid, dim3(256), 0, (hipStream_t)stream, g, io, tapbuf,
+                     tap_stride, shots, lj);
 }
 
 }  // namespace nw

@@ -30,6 +30,10 @@ struct Geo {
   const T* kappa[3];
   const T* zf[3];
   const T* zc[3];
+  // per axis, the cells [quiet[a][0], quiet[a][1]) whose damping is neutral (sponge 0, Cerjan
+  // taper >= 1, every cell without a layer): a tile inside that box steps with a = b = 1
+  int quiet[3][2];
+  int damp_kind;  // absorbing layer (nw_absorbing)
 };
 
 template <typename T>
@@ -92,11 +96,52 @@ struct AdjIO {
   double* grad;        // per-shot accumulators (or null)
 };
 
+// Temporally blocked forward steps (2-D float32): levels 1..levels of p^{n0+k}, k = 1..levels,
+// computed from p^{n0-1}, p^{n0} in one launch. Level k lands in out[k] when non-null.
+constexpr int kMaxDepth = 4;
+constexpr int kTapChunks = 8;  // receiver taps a workgroup scans per shot: 256 * kTapChunks
+
+struct BlockIO {
+  const float* m;
+  const float* p_prev;       // p^{n0-1}
+  const float* p;            // p^{n0}
+  float* out[kMaxDepth + 1];
+  int levels;                // 1 .. depth of the kernel
+  int n0;
+  int first_step;            // Cerjan: step n tapers its p^{n-1} when n > first_step
+  int n_src;                 // steps n < n_src inject the source
+  int src_taps;
+  const int32_t* src_idx;
+  const float* src_val;
+  const float* wavelet;
+  float* ck_prev;            // checkpoint slot receiving p^{n0-1}, p^{n0} (or null)
+  float* ck_cur;
+  // values of p at the receiver taps of levels 1 .. levels-1 (or null):
+  // tapbuf[((k-1) * n_shots + s) * tap_stride + r * taps + t]
+  float* tapbuf;
+  int tap_stride;
+  int taps;
+  const int32_t* rec_first;
+  const int32_t* rec_count;
+  const int32_t* rec_idx;
+};
+
 // fast 2-D float32 kernels (nefi_wave_2d.hip)
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype);
 void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
                  int ty, int spw, int xcd_map, void* stream);
 void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
                     int shots, int ty, int spw, int xcd_map, void* stream);
+// depth 2..kMaxDepth, ty 8/16/32; returns false for a shape it has no kernel for
+bool fast2d_block_supported(int ty, int depth);
+void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& io, int shots,
+                  int ty, int depth, int spw, int xcd_map, void* stream);
+// traces of levels 1..levels-1 from a BlockIO tap buffer; j[k-1] < 0 skips level k
+struct LevelRecords {
+  int j[kMaxDepth];
+};
+void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const float* tapbuf,
+                          int tap_stride, int rec_max, int shots, int levels,
+                          const LevelRecords& lj, void* stream);
 
 }  // namespace nw

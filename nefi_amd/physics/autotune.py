@@ -2,14 +2,17 @@
 
 For a plan the fast kernels cover, time a few forward steps and a few reverse steps for every
 candidate tile — rows of the strip one wavefront marches down × shots one workgroup advances
-in turn — and keep the fastest per kernel. Timing uses the library's own per-launch HIP events
-(``nw_profile_*``), so it measures the kernels alone. Results are cached in a JSON file keyed by
-device name, padded shape, shot count, order and absorbing layer
+in turn, and for the forward step the temporal blocking depth (steps advanced per launch) —
+and keep the fastest per kernel. Forward candidates are timed both as a plain sweep and as a
+sweep that keeps every level (the recompute of the reverse sweep), per step advanced. Timing
+uses the library's own per-launch HIP events (``nw_profile_*``), so it measures the kernels
+alone. Results are cached in a JSON file keyed by the candidate set's version, device name,
+padded shape, shot count, order and absorbing layer
 (``$NEFI_AMD_TILE_CACHE``, default ``~/.cache/nefi_amd/tiles.json``); ``NEFI_AMD_AUTOTUNE=0``
 keeps the library's defaults.
 
-This is the kernel autotuner the north star asks for; it is unrelated to nefi's
-hyper-parameter package ``nefi/autotune``.
+It tunes kernel launch shapes only and is unrelated to nefi's hyper-parameter package
+``nefi/autotune``.
 """
 
 from __future__ import annotations
@@ -26,6 +29,8 @@ from ..errors import NativeError
 ROWS = (4, 8, 16, 32, 64)
 LDS_ROWS = (8, 16, 32)
 GROUPS = (1, 2, 4, 8, 16)
+DEPTHS = (2, 3, 4)  # temporal blocking depths of the forward step
+BLOCK_STEPS = 12  # forward steps per timing: a multiple of every depth
 # below this many cell-updates per step a launch is latency-bound and every tile ties
 MIN_WORK = 1 << 18
 
@@ -56,26 +61,30 @@ def _store(table: dict) -> None:
 def plan_key(plan) -> str:
     h = plan.host
     name = torch.cuda.get_device_name(plan.device) if plan.device.type == "cuda" else "cpu"
-    return f"{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
+    return f"v2|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
 
 
 def candidates(kind: int, n_shots: int):
-    """(rows, shots per group, rows loaded per iteration, LDS tiles) to try for a kernel."""
+    """(rows, shots per group, rows loaded per iteration, LDS tiles, blocking depth) to try."""
     groups = [g for g in GROUPS if g <= n_shots]
-    out = [(r, g, u, 0) for r in ROWS for g in groups for u in ((1, 2) if kind == 0 else (1,))]
-    out += [(r, g, 1, 1) for r in LDS_ROWS for g in groups]
+    out = [(r, g, u, 0, 1) for r in ROWS for g in groups
+           for u in ((1, 2) if kind == 0 else (1,))]
+    out += [(r, g, 1, 1, 1) for r in LDS_ROWS for g in groups]
+    if kind == 0:
+        out += [(r, g, 1, 1, t) for r in LDS_ROWS for g in groups for t in DEPTHS]
     return out
 
 
-def _apply(plan, kind: int, rows: int, spw: int, rpi: int, lds: int) -> None:
+def _apply(plan, kind: int, rows: int, spw: int, rpi: int, lds: int, depth: int = 1) -> None:
     plan.set_tiles(kind, rows, spw)
     if kind == 0:
         plan.set_option(nat.OPT_ROWS_PER_ITER, rpi)
+        plan.set_option(nat.OPT_TIME_BLOCK, depth)
     plan.set_option(nat.OPT_LDS_TILES_STEP if kind == 0 else nat.OPT_LDS_TILES_ADJOINT, lds)
 
 
 def _time_kind(plan, kind: int, cand, steps: int) -> float:
-    """Average launch time (ms) of one kernel kind at one candidate."""
+    """Time (ms) per step of one kernel kind at one candidate."""
     P = plan
     _apply(P, kind, *cand)
     cells = P.host.cells
@@ -83,15 +92,17 @@ def _time_kind(plan, kind: int, cand, steps: int) -> float:
     state, psi = P.new_state()
     nat.profile_enable(True)  # (timed calls never go through graphs)
     if kind == 0:
+        keep = P.zeros((steps + 1) * P.keep_slot)
         P.forward(m, state, psi, 0, steps)
-        k = nat.K_STEP
+        P.forward(m, state, psi, steps, 2 * steps, keep=keep)
+        ms = nat.profile_read(nat.K_STEP)[0] + nat.profile_read(nat.K_STEPS)[0]
+        n = nat.profile_units(nat.K_STEP) + nat.profile_units(nat.K_STEPS)
     else:
         keep = P.zeros((steps + 1) * P.keep_slot)
         lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
         grad = P.zeros(P.grad_slices() * cells, torch.float64)
         P.adjoint(m, lam, psi, 0, steps, keep=keep, grad=grad)
-        k = nat.K_ADJOINT
-    ms, n = nat.profile_read(k)
+        ms, n = nat.profile_read(nat.K_ADJOINT)
     nat.profile_enable(False)
     return ms / max(1, n)
 
@@ -117,17 +128,19 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
             pick = {"xcd_map": xcd, "table": {}}
             for kind, name in ((0, "step"), (1, "adjoint")):
                 timings = {}
+                n = steps if kind == 1 else min(BLOCK_STEPS, max(1, h.n_steps))
                 for cand in candidates(kind, h.n_shots):
                     try:
                         _time_kind(plan, kind, cand, 1)  # warm the candidate once
-                        timings[cand] = _time_kind(plan, kind, cand, steps)
+                        timings[cand] = _time_kind(plan, kind, cand, n)
                     except NativeError:  # a variant this device refuses to launch
                         nat.profile_enable(False)
-                (rows, spw, rpi, lds), best = min(timings.items(), key=lambda kv: kv[1])
+                (rows, spw, rpi, lds, depth), best = min(timings.items(), key=lambda kv: kv[1])
                 pick[name] = {"rows": rows, "shots_per_group": spw, "rows_per_iter": rpi,
-                              "lds": lds, "ms": best}
-                pick["table"][name] = {f"{r}x{g}/{u}{'L' if l else ''}": round(t, 4)
-                                       for (r, g, u, l), t in timings.items()}
+                              "lds": lds, "depth": depth, "ms": best}
+                pick["table"][name] = {
+                    f"{r}x{g}/{u}{'L' if l else ''}{f'T{t}' if t > 1 else ''}": round(ms, 4)
+                    for (r, g, u, l, t), ms in timings.items()}
             # a gradient launches about two forward steps (sweep + recompute) per reverse step
             total = 2.0 * pick["step"]["ms"] + pick["adjoint"]["ms"]
             if best_total is None or total < best_total:
@@ -138,5 +151,5 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
     for kind, name in ((0, "step"), (1, "adjoint")):
         c = choice[name]
         _apply(plan, kind, c["rows"], c["shots_per_group"], c.get("rows_per_iter", 1),
-               c.get("lds", 0))
+               c.get("lds", 0), c.get("depth", 1))
     return choice

@@ -40,7 +40,7 @@ def _survey(absorbing="sponge", order=4, n=(70, 300), shots=5, n_t=60):
     return Domain(n, ext), geo, c, kw
 
 
-def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1):
+def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1):
     from nefi_amd import _native as nat
 
     x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
@@ -53,6 +53,7 @@ def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1):
         prop.plan.set_option(nat.OPT_LDS_TILES_STEP, lds[0])
         prop.plan.set_option(nat.OPT_LDS_TILES_ADJOINT, lds[1])
         prop.plan.set_option(nat.OPT_XCD_MAP, xcd)
+        prop.plan.set_option(nat.OPT_TIME_BLOCK, depth)
         y = op({"c": x})
     (g,) = torch.autograd.grad((y * w).sum(), x)
     return y.detach().cpu().double().numpy(), g.cpu().double().numpy()
@@ -77,6 +78,37 @@ def test_fast_kernels_match_generic_over_tiles(absorbing, order):
         print(f"{absorbing} o{order} tiles {tiles} lds {lds} xcd {xcd}: traces {ey:.2e}, "
               f"gradient {eg:.2e}", flush=True)
         assert ey < 1e-6 and eg < 1e-5
+
+
+@pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2),
+                                             ("sponge", 2)])
+def test_time_blocking_is_bit_identical(absorbing, order):
+    """Temporally blocked forward launches (depth 2-4) give exactly the traces and gradient of
+    one launch per step: uneven receivers sampled at intermediate levels, checkpoints every 13
+    steps (blocks cut short, down to one step), recompute sweeps keeping every level. The grid
+    is three column tiles wide, so tiles inside the layer-free box (stepped without damping or
+    bounds checks) and tiles at the grid edge both occur."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey(absorbing, order, n=(70, 600))
+    print(f"[case time-blocking {absorbing} o{order}]", flush=True)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(6))
+    for ty, spw, depth, xcd in ((16, 1, 2, 1), (8, 3, 3, 0), (32, 2, 4, 1), (16, 5, 4, 1),
+                                (8, 1, 4, 0), (32, 4, 3, 1)):
+        tiles = [(ty, spw), (16, 2)]
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=(1, 1), xcd=xcd, depth=1)
+        nat.profile_enable(True)
+        y, g = _grad(op, c, w, tiles=tiles, lds=(1, 1), xcd=xcd, depth=depth)
+        blocked = nat.profile_units(nat.K_STEPS)
+        nat.profile_enable(False)
+        ey, eg = rel(y, y0), rel(g, g0)
+        print(f"{absorbing} o{order} rows {ty} spw {spw} depth {depth} xcd {xcd}: "
+              f"{blocked} blocked steps, traces {ey:.1e}, gradient {eg:.1e}", flush=True)
+        assert blocked > 0
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
 
 
 def test_fast_path_against_oracle_uneven_shots():

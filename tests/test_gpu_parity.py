@@ -4,7 +4,7 @@ Every case of ``tests/golden`` runs through ``nefi_amd.physics.wave`` on the GPU
 
 * float64: traces and gradients must equal the reference's float64 results to round-off
   (rel-L2 ≤ 1e-10 / 1e-9): any difference in geometry, timing, stencil or adjoint shows here;
-* float32: traces within 1e-5 rel-L2 of the reference's own float32 traces (the north-star
+* float32: traces within 1e-5 rel-L2 of the reference's own float32 traces (the stated
   tolerance), gradients within 1e-4 of the float64 reference; the error of each float32 result
   against the float64 values is printed beside it.
 """
