@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "nefi_wave.h"
 #include "nefi_wave_internal.h"
@@ -680,30 +681,67 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
 // level and the writes of the next. Receiver taps that fall in the output tile are read out of
 // LDS at every intermediate level into a tap buffer, which a small kernel turns into trace
 // samples with k_sample's tap order; the last level is sampled from memory as usual.
-constexpr int kBlockWaves = 8;                  // wavefronts of one blocked workgroup
-constexpr int kBlockThreads = 64 * kBlockWaves;
-constexpr int kBlockTapChunks = kTapChunks * 256 / kBlockThreads;
+// wavefronts of one blocked workgroup: 16 for short tiles (more wavefronts per SIMD to hide
+// LDS latency), 8 for 32-row tiles (whose per-lane rows would not fit 128 registers)
+template <int TY>
+struct BlockWaves {
+  static constexpr int waves = TY >= 32 ? 8 : 16;
+  static constexpr int threads = 64 * waves;
+  static constexpr int tap_chunks = kTapChunks * 256 / threads;
+};
 
 template <int TY, int R, int T>
 struct BlockShape {
+  static constexpr int waves = BlockWaves<TY>::waves;
+  static constexpr int threads = BlockWaves<TY>::threads;
   static constexpr int hy = R * T;
   static constexpr int hx = (R * T + 3) / 4 * 4;
   static constexpr int out_w = 256 - 2 * hx;
-  static constexpr int rows = TY + 2 * hy;                         // region rows
-  static constexpr int mr = (rows + kBlockWaves - 1) / kBlockWaves;  // rows of one wavefront
-  static constexpr int crow = kBlockWaves * mr;                    // computed rows
+  static constexpr int rows = TY + 2 * hy;              // region rows
+  static constexpr int mr = (rows + waves - 1) / waves;  // rows of one wavefront
+  static constexpr int crow = waves * mr;               // computed rows
   static constexpr int lrows = crow + 2 * R;  // LDS rows: R zero rows above and below
   static constexpr int n4 = crow * (kLW / 4);  // float4 loads of one region
-  static constexpr int per_thread = (n4 + kBlockThreads - 1) / kBlockThreads;
+  static constexpr int per_thread = (n4 + threads - 1) / threads;
 };
+
+typedef __attribute__((address_space(1))) void gmem_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Workgroup barrier that orders LDS only: waits for this wavefront's LDS accesses, not for its
+// outstanding global-to-LDS copies (which __syncthreads' release fence would drain). The
+// memory clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Issue the loads of one region of a quiet tile (every element inside the grid) straight into
+// LDS rows R.. of `buf` (global_load_lds: lane l of a wavefront lands 16 bytes after lane l-1,
+// from the wavefront's base address). Completion is awaited with s_waitcnt before use.
+template <int TY, int R, int T>
+__device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, int P2,
+                                             float* buf) {
+  using S = BlockShape<TY, R, T>;
+  const int wbase = threadIdx.x & ~63;
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + S::threads * k;
+    if (S::threads * (k + 1) <= S::n4 || i < S::n4) {
+      const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+      const float* src = P + ((ry0 + row) * P2 + rx0 - 4 + 4 * c4);
+      float* dst = buf + R * kLW + 4 * (wbase + S::threads * k);
+      __builtin_amdgcn_global_load_lds((gmem_void*)src, (lds_void*)dst, 16, 0, 0);
+    }
+  }
+}
 
 // One tile of k4_steps. Q ("quiet") tiles lie inside the grid, margins included, and inside
 // the box of neutral damping: every bounds check is dead there and a = b = 1 (taper 1), so
 // they step with the arithmetic of the layer-free step, which is bit-identical to the
-// damped arithmetic at those coefficients (products by 1 and 2 p - 1 p' are exact).
+// damped arithmetic at those coefficients (products by 1 are exact). Quiet tiles also
+// double-buffer the region: the next shot's p^n streams into the second LDS buffer and its
+// p^{n-1} rows into registers while the current shot computes.
 template <int R, int ABS0, int TY, int T, bool Q>
 __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, int shots,
-                                        int spw, const Tile& tl, float* lp) {
+                                        int spw, const Tile& tl, float* lpa, float* lpb) {
   constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = BlockShape<TY, R, T>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
@@ -716,39 +754,65 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   const int c0 = wave * S::mr;  // first region row of this wavefront
   const float4 hx = (ABS != NW_NONE && inx) ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<ABS> dcol = damp4_col<ABS>(hx);
+  // wavelet samples of the block's steps (no vector loads inside the level loop)
+  float wl[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    const int n = io.n0 + k;
+    wl[k] = (io.src_taps > 0 && k < io.levels && n < io.n_src) ? io.wavelet[n] : 0.f;
+  }
   float4 mm[S::mr];
 #pragma unroll
   for (int k = 0; k < S::mr; ++k) {
     const int y = ry0 + c0 + k;
     mm[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.m + (y * P2 + x)) : f4(0.f);
   }
-  const int s_end = min(shots, (tl.grp + 1) * spw);
-  for (int s = tl.grp * spw; s < s_end; ++s) {
+  auto load_prev = [&](int s, float4* dst) {
     const int64_t base = (int64_t)s * g.cells;
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k) {
+      const int y = ry0 + c0 + k;
+      dst[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.p_prev + base + (y * P2 + x))
+                                                : f4(0.f);
+    }
+  };
+  const int s_first = tl.grp * spw;
+  const int s_end = min(shots, s_first + spw);
+  float4 pvn[Q ? S::mr : 1];
+  if constexpr (Q) {
+    region_async<TY, R, T>(io.p + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
+    load_prev(s_first, pvn);
+  }
+  // one shot; B selects the LDS buffer at compile time (even shots A, odd shots B), so the
+  // compiler sees that the asynchronous fill of the other buffer cannot alias this one's reads
+  auto shot = [&](auto bsel, int s) {
+    constexpr int B = decltype(bsel)::value;
+    const int64_t base = (int64_t)s * g.cells;
+    float* lp = (Q && B) ? lpb : lpa;
     float4 pv[S::mr], cur[S::mr];
-    {
+    if constexpr (Q) {
+      __builtin_amdgcn_s_waitcnt(0);  // this shot's region and p^{n-1} rows have landed
+      __syncthreads();                // ... for every wavefront; the other buffer is free
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) pv[k] = pvn[k];
+    } else {
       float4 v[S::per_thread];
       const float* P = io.p + base;
 #pragma unroll
       for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + kBlockThreads * k;
+        const int i = threadIdx.x + S::threads * k;
         const int row = i / (kLW / 4), c4 = i % (kLW / 4);
         const int gy = ry0 + row, gx = rx0 - 4 + 4 * c4;
-        const bool in = (kBlockThreads * (k + 1) <= S::n4 || i < S::n4);
-        const bool ok = in && (Q || (gy >= 0 && gy < D1 && gx >= 0 && gx < D2));
+        const bool in = (S::threads * (k + 1) <= S::n4 || i < S::n4);
+        const bool ok = in && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
         v[k] = ok ? ld4(P + (gy * P2 + gx)) : f4(0.f);
       }
-#pragma unroll
-      for (int k = 0; k < S::mr; ++k) {
-        const int y = ry0 + c0 + k;
-        pv[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.p_prev + base + (y * P2 + x))
-                                                 : f4(0.f);
-      }
+      load_prev(s, pv);
       __syncthreads();  // the previous shot's reads of lp are done
 #pragma unroll
       for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + kBlockThreads * k;
-        if (kBlockThreads * (k + 1) <= S::n4 || i < S::n4)
+        const int i = threadIdx.x + S::threads * k;
+        if (S::threads * (k + 1) <= S::n4 || i < S::n4)
           *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
       }
     }
@@ -760,12 +824,12 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
       src_rows |= sy >= 0 && sy < S::mr;
     }
     // receiver taps of this shot inside the output tile: LDS offset, or -1
-    int toff[kBlockTapChunks];
+    int toff[BlockWaves<TY>::tap_chunks];
     const int n_tap = io.tapbuf ? io.rec_count[s] * io.taps : 0;
     const int tap0 = io.tapbuf ? io.rec_first[s] * io.taps : 0;
 #pragma unroll
-    for (int c = 0; c < kBlockTapChunks; ++c) {
-      const int e = threadIdx.x + kBlockThreads * c;
+    for (int c = 0; c < BlockWaves<TY>::tap_chunks; ++c) {
+      const int e = threadIdx.x + S::threads * c;
       toff[c] = -1;
       if (e < n_tap) {
         const int idx = io.rec_idx[tap0 + e];
@@ -775,7 +839,13 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           toff[c] = (R + ty - ry0) * kLW + 4 + (tx - rx0);
       }
     }
-    __syncthreads();
+    if constexpr (!Q) __syncthreads();
+    if constexpr (Q) {
+      if (s + 1 < s_end) {  // stream the next shot in while this one computes
+        region_async<TY, R, T>(io.p + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
+        load_prev(s + 1, pvn);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
     if (io.ck_prev) {
@@ -791,7 +861,9 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     for (int lvl = 1; lvl <= io.levels; ++lvl) {
       const int n = io.n0 + lvl - 1;
       const bool src = src_rows && n < io.n_src;
-      const float w_n = src ? io.wavelet[n] : 0.f;
+      float w_n = 0.f;
+#pragma unroll
+      for (int k = 0; k < T; ++k) w_n = (k == lvl - 1) ? wl[k] : w_n;
       const int tpv = n > io.first_step;
       float4 w[2 * R + 1];
 #pragma unroll
@@ -832,16 +904,16 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
         for (int j = 0; j < 2 * R; ++j) w[j] = w[j + 1];
       }
       if (lvl < io.levels) {
-        __syncthreads();  // every read of the previous level is done
+        lds_barrier();  // every read of the previous level is done
 #pragma unroll
         for (int k = 0; k < S::mr; ++k)
           *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = pv[k];
-        __syncthreads();
+        lds_barrier();
         if (io.tapbuf) {
           float* tb = io.tapbuf + ((int64_t)(lvl - 1) * shots + s) * io.tap_stride;
 #pragma unroll
-          for (int c = 0; c < kBlockTapChunks; ++c)
-            if (toff[c] >= 0) tb[threadIdx.x + kBlockThreads * c] = lp[toff[c]];
+          for (int c = 0; c < BlockWaves<TY>::tap_chunks; ++c)
+            if (toff[c] >= 0) tb[threadIdx.x + S::threads * c] = lp[toff[c]];
         }
       }
       float* out = io.out[lvl];
@@ -860,29 +932,36 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
         pv[k] = t;
       }
     }
+  };
+  for (int s = s_first; s < s_end; s += 2) {
+    shot(std::integral_constant<int, 0>{}, s);
+    if (s + 1 < s_end) shot(std::integral_constant<int, 1>{}, s + 1);
   }
 }
 
 template <int R, int ABS, int TY, int T>
-__global__ void __launch_bounds__(kBlockThreads) k4_steps(Geo<float> g, BlockIO io, int shots,
+__global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g, BlockIO io, int shots,
                                                           int spw, int xcd_map) {
   using S = BlockShape<TY, R, T>;
-  __shared__ __align__(16) float lp[S::lrows * kLW];
+  __shared__ __align__(16) float lpa[S::lrows * kLW];
+  __shared__ __align__(16) float lpb[S::lrows * kLW];
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
   const Tile tl = tile_of(div_up(D2, S::out_w), div_up(D1, TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
-  for (int i = threadIdx.x; i < R * kLW; i += kBlockThreads) {
-    lp[i] = 0.f;
-    lp[(S::lrows - R) * kLW + i] = 0.f;
+  for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
+    lpa[i] = 0.f;
+    lpa[(S::lrows - R) * kLW + i] = 0.f;
+    lpb[i] = 0.f;
+    lpb[(S::lrows - R) * kLW + i] = 0.f;
   }
   const int ry0 = tl.ry * TY - S::hy, rx0 = tl.cx * S::out_w - S::hx;
   const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
                      ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&
                      rx0 >= g.quiet[2][0] && rx0 + 256 <= g.quiet[2][1];
   if (quiet)
-    k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lp);
+    k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lpa, lpb);
   else
-    k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lp);
+    k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lpa, lpb);
 }
 
 // trace samples of the intermediate levels of a blocked launch, from its tap buffer: the
@@ -1016,7 +1095,7 @@ static void launch_steps_t(const Geo<float>& g, const BlockIO& io, int shots, in
   const int tiles =
       div_up((int)g.D[2], S::out_w) * div_up((int)g.D[1], TY) * div_up(shots, spw);
   hipLaunchKernelGGL((k4_steps<R, ABS, TY, T>), dim3((unsigned)(kXcds * div_up(tiles, kXcds))),
-                     dim3(kBlockThreads), 0, st, g, io, shots, spw, xm & 1);
+                     dim3(S::threads), 0, st, g, io, shots, spw, xm & 1);
 }
 
 template <int R, int ABS, int TY>

@@ -60,7 +60,13 @@ def _store(table: dict) -> None:
 
 def plan_key(plan) -> str:
     h = plan.host
-    name = torch.cuda.get_device_name(plan.device) if plan.device.type == "cuda" else "cpu"
+    if plan.device.type == "cuda":
+        # the architecture name and CU count, which (unlike the marketing name) profilers
+        # that wrap the process still report
+        props = torch.cuda.get_device_properties(plan.device)
+        name = f"{getattr(props, 'gcnArchName', '') or props.name}/{props.multi_processor_count}"
+    else:
+        name = "cpu"
     return f"v2|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
 
 
