@@ -37,7 +37,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"step": 16.0, "steps_blocked": 16.0, "adjoint": 28.0}
+BYTES_PER_UPDATE = {"step": 16.0, "steps_blocked": 16.0, "adjoint": 28.0,
+                    "adjoint_blocked": 28.0}
 
 
 def survey(n=1024, shots=64, n_t=4001, seed=0):
@@ -294,7 +295,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
     kinds = {"step": nat.K_STEP, "steps_blocked": nat.K_STEPS, "adjoint": nat.K_ADJOINT,
-             "sample": nat.K_SAMPLE, "inject": nat.K_INJECT}
+             "adjoint_blocked": nat.K_ADJOINT_STEPS, "sample": nat.K_SAMPLE,
+             "inject": nat.K_INJECT}
     # timed region 1: every kernel launch bracketed by HIP events (calls run eagerly)
     nat.profile_enable(True)
     torch.cuda.synchronize()
@@ -330,16 +332,22 @@ def main() -> int:
     tiles["lds"] = [plan.option(nat.OPT_LDS_TILES_STEP), plan.option(nat.OPT_LDS_TILES_ADJOINT)]
     tiles["hip_graph"] = plan.option(nat.OPT_GRAPH)
     tiles["time_block"] = plan.option(nat.OPT_TIME_BLOCK)
-    dom_kind = max(("step", "steps_blocked", "adjoint"), key=lambda k: timing[k][0])
-    rows_t = tiles["adjoint" if dom_kind == "adjoint" else "step"]["rows"]
-    lds_t = tiles["lds"][1 if dom_kind == "adjoint" else 0]
+    tiles["time_block_adjoint"] = plan.option(nat.OPT_TIME_BLOCK_ADJOINT)
+    tiles["keep_laplacian"] = plan.option(nat.OPT_KEEP_LAP)
+    dom_kind = max(("step", "steps_blocked", "adjoint", "adjoint_blocked"),
+                   key=lambda k: timing[k][0])
+    reverse = dom_kind.startswith("adjoint")
+    rows_t = tiles["adjoint" if reverse else "step"]["rows"]
+    lds_t = tiles["lds"][1 if reverse else 0]
     fam = "k3" if lds_t else "k2"
-    depth = tiles["time_block"]
+    blocked = dom_kind in ("steps_blocked", "adjoint_blocked")
+    depth = tiles["time_block_adjoint" if reverse else "time_block"]
     kname = {"step": f"{fam}_step" if rows_t else "k_step",
              "steps_blocked": "k4_steps",
-             "adjoint": f"{fam}_adjoint" if rows_t else "k_adjoint"}[dom_kind]
+             "adjoint": f"{fam}_adjoint" if rows_t else "k_adjoint",
+             "adjoint_blocked": "k5_adjoint"}[dom_kind]
     kname += f"<order 4, sponge{', rows %d' % rows_t if rows_t else ''}"
-    kname += f", depth {depth}>" if dom_kind == "steps_blocked" else ">"
+    kname += f", depth {depth}>" if blocked else ">"
     tot_ms, launches = timing[dom_kind]
     # (under a counter-collecting profiler the library's events record nothing)
     avg_s = tot_ms / 1e3 / launches if launches and tot_ms > 0 else float("nan")
@@ -351,7 +359,7 @@ def main() -> int:
     inst = f"nw::{kname.split('<')[0]}<{r_code}, {a_code}, {rows_t}"
     if fam == "k2" and dom_kind == "step":
         inst += f", {tiles['rows_per_iter']}"
-    if dom_kind == "steps_blocked":
+    if blocked:
         inst += f", {depth}"
     traffic, traffic_src = measured_traffic(inst + ">")
     roofline = {

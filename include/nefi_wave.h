@@ -182,7 +182,19 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   once (results bit-identical to T single steps). Used when the forward tiles have rows of
  *   8, 16 or 32 and, for calls that record traces, a shot has at most 2048 receiver taps;
  *   other calls step one by one. Setting T > 1 allocates two wavefields and a small tap
- *   buffer held by the plan. */
+ *   buffer held by the plan.
+ * NW_OPT_KEEP_LAP: 1 (default) lets keep buffers hold the Laplacian of p^n instead of p^n
+ *   whenever both kernel kinds run the fast 2-D kernels (reads back as 1 only then): the
+ *   forward kernels write it as they step, and the reverse kernels read it pointwise for the
+ *   imaging term (same values, no stencil). A keep sweep feeding such a reverse sweep over
+ *   steps [n0, n1) must then run forward over [n0, n1) (not n1 - 1). 0 keeps p^n.
+ * NW_OPT_TIME_BLOCK_ADJOINT: temporal blocking depth of the fast reverse step (1 default,
+ *   2..4): up to that many reverse steps per launch, with the residual injections of recorded
+ *   steps and the imaging sum inside the launch (lambda bit-identical to single steps; the
+ *   float64 imaging sums are added in another order). Used when the reverse tiles have rows of
+ *   8 or 16, the imaging term (if any) reads kept Laplacians (NW_OPT_KEEP_LAP) and a shot has
+ *   at most 2048 injection cells; other calls step one by one. Shares the plan's blocking
+ *   wavefields with NW_OPT_TIME_BLOCK. */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,
@@ -190,7 +202,9 @@ enum nw_option {
   NW_OPT_LDS_TILES_ADJOINT = 3,
   NW_OPT_GRAPH = 4,
   NW_OPT_GRAPH_COUNT = 5,
-  NW_OPT_TIME_BLOCK = 6
+  NW_OPT_TIME_BLOCK = 6,
+  NW_OPT_KEEP_LAP = 7,
+  NW_OPT_TIME_BLOCK_ADJOINT = 8
 };
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
@@ -255,7 +269,8 @@ int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* value
  * by HIP events on its own stream, and calls run eagerly even in graph mode. nw_profile_read
  * waits for the recorded events and returns the summed duration (ms) and launch count of one
  * kernel kind since profiling was enabled: 0 = forward step, 1 = reverse step,
- * 2 = receiver sample, 3 = residual inject, 4 = temporally blocked forward steps.
+ * 2 = receiver sample, 3 = residual inject, 4 = temporally blocked forward steps,
+ * 5 = temporally blocked reverse steps.
  * nw_profile_units returns the time steps those launches advanced (for kind 2, the time
  * samples recorded). Enabling clears earlier records. Process-wide; not thread-safe. */
 enum nw_kernel_kind {
@@ -263,7 +278,8 @@ enum nw_kernel_kind {
   NW_K_ADJOINT = 1,
   NW_K_SAMPLE = 2,
   NW_K_INJECT = 3,
-  NW_K_STEPS = 4
+  NW_K_STEPS = 4,
+  NW_K_ADJOINT_STEPS = 5
 };
 int32_t nw_profile_enable(int32_t on);
 int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches);

@@ -163,7 +163,7 @@ def lib() -> ctypes.CDLL:
     return handle
 
 
-K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT, K_STEPS = 0, 1, 2, 3, 4
+K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT, K_STEPS, K_ADJOINT_STEPS = 0, 1, 2, 3, 4, 5
 OPT_XCD_MAP = 0
 OPT_ROWS_PER_ITER = 1
 OPT_LDS_TILES_STEP = 2
@@ -171,6 +171,8 @@ OPT_LDS_TILES_ADJOINT = 3
 OPT_GRAPH = 4
 OPT_GRAPH_COUNT = 5
 OPT_TIME_BLOCK = 6
+OPT_KEEP_LAP = 7
+OPT_TIME_BLOCK_ADJOINT = 8
 
 
 def profile_enable(on: bool) -> None:

@@ -246,16 +246,32 @@ __global__ void __launch_bounds__(256) k_sample(Geo<T> g, SampleIO<T> io) {
 
 // lam^n += R^T g_n for one recorded step (deterministic: one thread per distinct cell)
 template <typename T>
+__device__ __forceinline__ T inject_sum(const InjectIO<T>& io, int s, int uu, int j) {
+  const int64_t row0 = io.trace_first[s];
+  T acc = T(0);
+  for (int e = io.inj_off[uu]; e < io.inj_off[uu + 1]; ++e)
+    acc += io.inj_w[e] * io.residual[(row0 + io.inj_rec[e]) * io.n_t + j];
+  return acc;
+}
+
+template <typename T>
 __global__ void __launch_bounds__(256) k_inject(Geo<T> g, InjectIO<T> io) {
   const int s = blockIdx.y;
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= io.inj_count[s]) return;
   const int uu = io.inj_first[s] + u;
-  const int64_t row0 = io.trace_first[s];
-  T acc = T(0);
-  for (int e = io.inj_off[uu]; e < io.inj_off[uu + 1]; ++e)
-    acc += io.inj_w[e] * io.residual[(row0 + io.inj_rec[e]) * io.n_t + io.j];
-  io.lam[(int64_t)s * g.cells + io.inj_cell[uu]] += acc;
+  io.lam[(int64_t)s * g.cells + io.inj_cell[uu]] += inject_sum(io, s, uu, io.j);
+}
+
+// The sums k_inject adds, for the recorded levels of a blocked reverse launch:
+// out[(k * n_shots + s) * inj_max + u] for level k = blockIdx.z (j[k] < 0: not recorded).
+__global__ void __launch_bounds__(256) k_inject_values(InjectIO<float> io, float* out,
+                                                       int n_shots, int inj_max,
+                                                       LevelRecords lj) {
+  const int s = blockIdx.y, k = blockIdx.z;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lj.j[k] < 0 || u >= io.inj_count[s]) return;
+  out[((int64_t)k * n_shots + s) * inj_max + u] = inject_sum(io, s, io.inj_first[s] + u, lj.j[k]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -526,6 +542,11 @@ struct nw_plan {
   int tblock;
   float* tb_state[2];
   float* tapbuf;
+  int keep_lap_opt;  // NW_OPT_KEEP_LAP
+  // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth and the
+  // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
+  int tblock_adj;
+  float* injv;
   // hipGraph capture of whole nw_forward / nw_adjoint calls (NW_OPT_GRAPH)
   int graph_mode;
   hipStream_t capture_stream;
@@ -606,9 +627,8 @@ static int launch_sample(nw_plan* P, const T* p, T* traces, int j, hipStream_t s
 }
 
 template <typename T>
-static int launch_inject(nw_plan* P, T* lam, const T* residual, int j, hipStream_t st) {
+static InjectIO<T> inject_io(const nw_plan* P, T* lam, const T* residual, int j) {
   const nw_desc& d = P->d;
-  if (d.inj_max <= 0) return NW_OK;
   InjectIO<T> io{};
   io.lam = lam;
   io.residual = residual;
@@ -621,6 +641,14 @@ static int launch_inject(nw_plan* P, T* lam, const T* residual, int j, hipStream
   io.inj_rec = d.inj_rec;
   io.inj_w = (const T*)d.inj_w;
   io.trace_first = d.trace_first;
+  return io;
+}
+
+template <typename T>
+static int launch_inject(nw_plan* P, T* lam, const T* residual, int j, hipStream_t st) {
+  const nw_desc& d = P->d;
+  if (d.inj_max <= 0) return NW_OK;
+  const InjectIO<T> io = inject_io<T>(P, lam, residual, j);
   dim3 grid((d.inj_max + 255) / 256, d.n_shots);
   const int pid = prof_begin(NW_K_INJECT, st);
   hipLaunchKernelGGL(k_inject<T>, grid, dim3(256), 0, st, geo<T>(P), io);
@@ -728,6 +756,13 @@ static int hand_back(nw_plan* P, void** state, T* s0, T* s1, T* prev, T* cur, hi
   return NW_OK;
 }
 
+// Keep buffers hold the Laplacian of p^n (instead of p^n) when both fast 2-D kernel families
+// run: the forward kernels compute it anyway, and the reverse kernels' imaging term then reads
+// it pointwise instead of re-applying the stencil to a p tile.
+static bool keep_lap(const nw_plan* P) {
+  return P->d.dtype == NW_F32 && P->keep_lap_opt && P->rows[0] > 0 && P->rows[1] > 0;
+}
+
 // The blocked sweep applies when the plan asks for it and the forward tiles have a blocked
 // kernel; a sweep recording traces also needs every shot's receiver taps in one scan.
 static bool use_blocked(const nw_plan* P, bool traces) {
@@ -753,7 +788,8 @@ static int forward_blocked(nw_plan* P, const float* m, void** state, int n_begin
     return (float*)((char*)keep + (int64_t)(n - keep_begin) * keep_slot);
   };
   const bool keeping = keep != nullptr;
-  if (keeping && keep_begin == n_begin)
+  const bool klap = keeping && keep_lap(P);
+  if (keeping && !klap && keep_begin == n_begin)
     NW_HIP(hipMemcpyAsync(kslot(n_begin), s1, P->shot_bytes, hipMemcpyDeviceToDevice, st));
   const bool taps = traces && d.rec_max > 0;
   float* prev = s0;
@@ -781,7 +817,9 @@ static int forward_blocked(nw_plan* P, const float* m, void** state, int n_begin
       io.ck_prev = slot;
       io.ck_cur = (float*)((char*)slot + P->shot_bytes);
     }
-    if (keeping) {
+    if (klap)
+      for (int k = 1; k <= L; ++k) io.lap_out[k] = kslot(n + k - 1);
+    if (keeping && !klap) {
       for (int k = 1; k <= L; ++k) io.out[k] = kslot(n + k);
     } else {
       float* spare[2] = {nullptr, nullptr};
@@ -862,7 +900,8 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
       }
   }
   auto kslot = [&](int n) { return (T*)((char*)keep + (int64_t)(n - keep_begin) * keep_slot); };
-  const bool keeping = keep != nullptr;
+  const bool klap = keep != nullptr && keep_lap(P);
+  const bool keeping = keep != nullptr && !klap;  // keep slots carry the state chain
   if (keeping && keep_begin == n_begin) {
     NW_HIP(hipMemcpyAsync(kslot(n_begin), s1, P->shot_bytes, hipMemcpyDeviceToDevice, st));
     if (pml) {
@@ -898,6 +937,7 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
       io.psi_new[a] = psi_next[a];
     }
     io.taper_prev = n > first_step;
+    if (klap) io.lap_out = kslot(n);
     if (d.src_taps > 0 && n < d.n_steps) {
       io.src_taps = d.src_taps;
       io.src_idx = d.src_idx;
@@ -946,12 +986,104 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
   return NW_OK;
 }
 
+// The blocked reverse sweep applies when the plan asks for it, the reverse tiles have a
+// blocked kernel, the imaging term (if any) reads kept Laplacians, and a shot's injection cells
+// fit one scan of the workgroup.
+static bool use_blocked_adj(const nw_plan* P, const void* keep, const void* residual) {
+  const nw_desc& d = P->d;
+  return d.dtype == NW_F32 && P->tblock_adj > 1 && P->rows[1] > 0 && P->tb_state[0] &&
+         P->injv && fast2d_adj_block_supported(P->rows[1], P->tblock_adj) &&
+         (!keep || keep_lap(P)) && (!residual || d.inj_max <= 256 * kTapChunks);
+}
+
+// Reverse sweep in blocks of up to tblock_adj steps (2-D float32 fast path). Each block's two
+// outputs go to the two of {lam pair, tb_state} the block does not read.
+static int adjoint_blocked(nw_plan* P, const float* ma, void** lam, int n_begin, int n_end,
+                           const float* keep, int keep_begin, const float* residual,
+                           int inject_end, double* grad, int first_step, hipStream_t st) {
+  const nw_desc& d = P->d;
+  const int64_t keep_slot = nw_keep_slot_bytes(P);
+  auto kslot = [&](int n) {
+    return (const float*)((const char*)keep + (int64_t)(n - keep_begin) * keep_slot);
+  };
+  float* a0 = (float*)lam[0];
+  float* a1 = (float*)lam[1];
+  float* const bufs[4] = {a0, a1, P->tb_state[0], P->tb_state[1]};
+  float* L1 = a0;  // lambda^{n}
+  float* L2 = a1;  // lambda^{n+1}
+  if (residual && inject_end && recorded(d, n_end))
+    NW_TRY(launch_inject<float>(P, L1, residual, n_end / d.substeps, st));
+  for (int n = n_end; n > n_begin;) {
+    const int L = std::min(P->tblock_adj, n - n_begin);
+    AdjBlockIO io{};
+    io.ma = ma;
+    io.lam1 = L1;
+    io.lam2 = L2;
+    io.levels = L;
+    io.nh = n;
+    io.first_step = first_step;
+    if (keep) {
+      io.grad = grad;
+      for (int k = 1; k <= L; ++k) io.lap[k] = kslot(n - k);
+    }
+    float* spare[2] = {nullptr, nullptr};
+    int ns = 0;
+    for (float* b : bufs)
+      if (b != L1 && b != L2 && ns < 2) spare[ns++] = b;
+    if (L >= 2) {
+      io.out[L - 1] = spare[0];
+      io.out[L] = spare[1];
+    } else {
+      io.out[1] = spare[0];
+    }
+    if (residual && d.inj_max > 0) {
+      LevelRecords lj;
+      int nrec = 0;
+      for (int k = 0; k < kMaxDepth; ++k) lj.j[k] = -1;
+      for (int k = 1; k <= L; ++k)
+        if (recorded(d, n - k)) {
+          lj.j[k - 1] = (n - k) / d.substeps;
+          io.inj_rec[k] = 1;
+          ++nrec;
+        }
+      if (nrec > 0) {
+        const InjectIO<float> iio = inject_io<float>(P, nullptr, residual, 0);
+        const dim3 grid((d.inj_max + 255) / 256, d.n_shots, L);
+        const int pid = prof_begin(NW_K_INJECT, st, nrec);
+        hipLaunchKernelGGL(k_inject_values, grid, dim3(256), 0, st, iio, P->injv, d.n_shots,
+                           d.inj_max, lj);
+        prof_end(pid, st);
+        NW_TRY(check_launch("k_inject_values"));
+        io.injv = P->injv;
+        io.inj_max = d.inj_max;
+        io.inj_first = d.inj_first;
+        io.inj_count = d.inj_count;
+        io.inj_cell = d.inj_cell;
+      }
+    }
+    const int pid = prof_begin(NW_K_ADJOINT_STEPS, st, L);
+    fast2d_adjoint_steps(P->gf, d.order, d.absorbing, io, d.n_shots, P->rows[1], P->tblock_adj,
+                         P->spw[1], P->xcd_map, st);
+    prof_end(pid, st);
+    NW_TRY(check_launch("k5_adjoint"));
+    L2 = L >= 2 ? io.out[L - 1] : L1;
+    L1 = io.out[L];
+    n -= L;
+  }
+  return hand_back<float>(P, lam, a0, a1, L1, L2, st);
+}
+
 template <typename T>
 static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** psib, int n_begin,
                         int n_end, const T* keep, int keep_begin, const T* residual,
                         int inject_end, double* grad, int first_step, hipStream_t st) {
   (void)m;
   const nw_desc& d = P->d;
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_blocked_adj(P, keep, residual))
+      return adjoint_blocked(P, ma, lam, n_begin, n_end, keep, keep_begin, residual, inject_end,
+                             grad, first_step, st);
+  }
   const bool pml = d.absorbing == NW_PML;
   const int64_t keep_slot = nw_keep_slot_bytes(P);
   T* L1 = (T*)lam[0];
@@ -980,7 +1112,10 @@ static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** 
     io.lam0 = L2;
     io.taper_next = (n + 1) > first_step;
     if (keep) {
-      io.p_n = kslot(n);
+      if (keep_lap(P))
+        io.lap_n = kslot(n);
+      else
+        io.p_n = kslot(n);
       io.grad = grad;
       if (pml) {
         T* lo[3];
@@ -1178,6 +1313,9 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tblock = 1;
   p->tb_state[0] = p->tb_state[1] = nullptr;
   p->tapbuf = nullptr;
+  p->keep_lap_opt = 0;
+  p->tblock_adj = 1;
+  p->injv = nullptr;
   p->graph_mode = 0;
   p->capture_stream = nullptr;
   p->graph_serial = 0;
@@ -1196,8 +1334,8 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   if (!plan) return NW_OK;
   drop_graphs(plan);
   if (plan->capture_stream) hipStreamDestroy(plan->capture_stream);
-  for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf})
-    if (b) hipFree(b);
+  for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
+    if (b) (void)hipFree(b);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)
     if (nw::g_plans[i] == plan) {
       nw::g_plans.erase(nw::g_plans.begin() + i);
@@ -1264,7 +1402,11 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     P->lds_tiles[k] = value != 0;
     return NW_OK;
   }
-  if (option == NW_OPT_TIME_BLOCK) {
+  if (option == NW_OPT_KEEP_LAP) {
+    P->keep_lap_opt = value != 0;
+    return NW_OK;
+  }
+  if (option == NW_OPT_TIME_BLOCK || option == NW_OPT_TIME_BLOCK_ADJOINT) {
     if (value < 1 || value > kMaxDepth)
       return fail(NW_ERR_ARG, "time block depth must be 1.." + std::to_string(kMaxDepth));
     if (value > 1 && !P->tb_state[0]) {
@@ -1272,11 +1414,13 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
         return fail(NW_ERR_UNSUPPORTED, "time blocking needs the fast 2-D float32 kernels");
       const int64_t taps = (int64_t)(kMaxDepth - 1) * P->d.n_shots *
                            std::max(1, P->d.rec_max * P->d.rec_taps);
+      const int64_t inj = (int64_t)kMaxDepth * P->d.n_shots * std::max(1, P->d.inj_max);
       for (int k = 0; k < 2; ++k)
         NW_HIP(hipMalloc((void**)&P->tb_state[k], (size_t)P->shot_bytes));
       NW_HIP(hipMalloc((void**)&P->tapbuf, (size_t)taps * sizeof(float)));
+      NW_HIP(hipMalloc((void**)&P->injv, (size_t)inj * sizeof(float)));
     }
-    P->tblock = value;
+    (option == NW_OPT_TIME_BLOCK ? P->tblock : P->tblock_adj) = value;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_set_option: unknown option");
@@ -1306,6 +1450,14 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_TIME_BLOCK) {
     *value = P->tblock;
+    return NW_OK;
+  }
+  if (option == NW_OPT_TIME_BLOCK_ADJOINT) {
+    *value = P->tblock_adj;
+    return NW_OK;
+  }
+  if (option == NW_OPT_KEEP_LAP) {
+    *value = keep_lap(P) ? 1 : 0;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

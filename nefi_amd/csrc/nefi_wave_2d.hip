@@ -236,6 +236,7 @@ __device__ __forceinline__ void step_row(const Geo<float>& g, const StepIO<float
   }
   if (ABS != NW_NONE) o = mul4(dm.a, o);
   st4(io.p_next + base + ro, o);
+  if (io.lap_out) st4(io.lap_out + base + ro, lap);
   if (io.ck_prev) {
     st4(io.ck_prev + base + ro, pp);
     st4(io.ck_cur + base + ro, pc);
@@ -327,6 +328,7 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
     const int64_t base = (int64_t)s * g.cells;
     const float* L1 = io.lam1 + base;
     const float* Pn = io.p_n ? io.p_n + base : nullptr;
+    const float* Ln = io.lap_n ? io.lap_n + base : nullptr;
     // window of q = (m a_eff) * lambda^{n+1}, formed as the rows arrive
     float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
@@ -345,7 +347,7 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
       const float2 el = edge2(L1, y, D2, P2, x, lane);
       const float2 em = edge2(MA, y, D2, P2, x, lane);
       const float2 ep = Pn ? edge2(Pn, y, D2, P2, x, lane) : make_float2(0.f, 0.f);
-      double2* gp = (G && Pn && act) ? reinterpret_cast<double2*>(G + ro) : nullptr;
+      double2* gp = (G && (Pn || Ln) && act) ? reinterpret_cast<double2*>(G + ro) : nullptr;
       double2 g0 = make_double2(0.0, 0.0), g1 = g0;
       if (gp) {
         g0 = gp[0];
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
             adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
         st4(io.lam0 + base + ro, o);
         if (gp) {
-          const float4 lapp = stencil4<R>(g, pw, pnb.x, pnb.y, pnb.z, pnb.w);
+          const float4 lapp = Ln ? ld4(Ln + ro) : stencil4<R>(g, pw, pnb.x, pnb.y, pnb.z, pnb.w);
           g0.x = __builtin_fma((double)rb.x, (double)lapp.x, g0.x);
           g0.y = __builtin_fma((double)rb.y, (double)lapp.y, g0.y);
           g1.x = __builtin_fma((double)rb.z, (double)lapp.z, g1.x);
@@ -509,6 +511,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
         }
         if (ABS != NW_NONE) o = mul4(dm.a, o);
         st4(io.p_next + base + ro, o);
+        if (io.lap_out) st4(io.lap_out + base + ro, lap);
         if (io.ck_prev) {
           st4(io.ck_prev + base + ro, pp[k]);
           st4(io.ck_cur + base + ro, pc);
@@ -537,7 +540,9 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   const bool act = Q || x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<ABS> dcol = damp4_col<ABS>(hx);
-  const bool imaging = io.p_n != nullptr && io.grad != nullptr;
+  const bool kept_lap = io.lap_n != nullptr;
+  const bool imaging = (io.p_n != nullptr || kept_lap) && io.grad != nullptr;
+  const bool p_tile = imaging && !kept_lap;
   const int tpn = io.taper_next;
   // tile loads: every element inside the grid for Q tiles
   auto load_tile = [&](const float* src, float4* v) {
@@ -572,31 +577,32 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
       for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
       tile_store<TY, R>(lq, vl);
     }
-    if (imaging) {
+    if (p_tile) {
       float4 vp[S::per_thread];
       load_tile(io.p_n + base, vp);
       tile_store<TY, R>(lp, vp);
     }
-    float4 lc[S::my_rows], l2c[S::my_rows];
+    float4 lc[S::my_rows], l2c[S::my_rows], lpn[S::my_rows];
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
       const bool ok = Q || (act && y < D1);
       lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // just loaded into the tile: cache hit
       l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+      lpn[k] = (ok && imaging && kept_lap) ? ld4(io.lap_n + base + (y * P2 + x)) : f4(0.f);
     }
     __syncthreads();
     float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
       qw[k] = lrow(lq, r0 + k, lane);
-      pw[k] = imaging ? lrow(lp, r0 + k, lane) : f4(0.f);
+      pw[k] = p_tile ? lrow(lp, r0 + k, lane) : f4(0.f);
     }
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
       qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
-      pw[2 * R] = imaging ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
+      pw[2 * R] = p_tile ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
       if (Q || (act && y < D1)) {
         const float4 qn = lnb(lq, r0 + k + R, lane);
         const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
@@ -610,8 +616,11 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
             adj1<ABS>(lc[k].w, l2c[k].w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
         st4(io.lam0 + base + (y * P2 + x), o);
         if (imaging) {
-          const float4 pn = lnb(lp, r0 + k + R, lane);
-          const float4 lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
+          float4 lapp = lpn[k];
+          if (p_tile) {
+            const float4 pn = lnb(lp, r0 + k + R, lane);
+            lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
+          }
           ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
           ga[k].y = __builtin_fma((double)rb.y, (double)lapp.y, ga[k].y);
           gb[k].x = __builtin_fma((double)rb.z, (double)lapp.z, gb[k].x);
@@ -690,10 +699,10 @@ struct BlockWaves {
   static constexpr int tap_chunks = kTapChunks * 256 / threads;
 };
 
-template <int TY, int R, int T>
+template <int TY, int R, int T, int W = BlockWaves<TY>::waves>
 struct BlockShape {
-  static constexpr int waves = BlockWaves<TY>::waves;
-  static constexpr int threads = BlockWaves<TY>::threads;
+  static constexpr int waves = W;
+  static constexpr int threads = 64 * W;
   static constexpr int hy = R * T;
   static constexpr int hx = (R * T + 3) / 4 * 4;
   static constexpr int out_w = 256 - 2 * hx;
@@ -716,10 +725,9 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // Issue the loads of one region of a quiet tile (every element inside the grid) straight into
 // LDS rows R.. of `buf` (global_load_lds: lane l of a wavefront lands 16 bytes after lane l-1,
 // from the wavefront's base address). Completion is awaited with s_waitcnt before use.
-template <int TY, int R, int T>
+template <typename S, int R>
 __device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, int P2,
                                              float* buf) {
-  using S = BlockShape<TY, R, T>;
   const int wbase = threadIdx.x & ~63;
 #pragma unroll
   for (int k = 0; k < S::per_thread; ++k) {
@@ -780,7 +788,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   const int s_end = min(shots, s_first + spw);
   float4 pvn[Q ? S::mr : 1];
   if constexpr (Q) {
-    region_async<TY, R, T>(io.p + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
+    region_async<S, R>(io.p + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
     load_prev(s_first, pvn);
   }
   // one shot; B selects the LDS buffer at compile time (even shots A, odd shots B), so the
@@ -842,7 +850,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     if constexpr (!Q) __syncthreads();
     if constexpr (Q) {
       if (s + 1 < s_end) {  // stream the next shot in while this one computes
-        region_async<TY, R, T>(io.p + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
+        region_async<S, R>(io.p + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
         load_prev(s + 1, pvn);
       }
     }
@@ -865,6 +873,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
 #pragma unroll
       for (int k = 0; k < T; ++k) w_n = (k == lvl - 1) ? wl[k] : w_n;
       const int tpv = n > io.first_step;
+      float* lap_out = io.lap_out[lvl];
       float4 w[2 * R + 1];
 #pragma unroll
       for (int j = 0; j < 2 * R; ++j) w[j] = lrow(lp, c0 + j, lane);
@@ -876,6 +885,9 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
         if (Q || (y >= 0 && y < D1)) {  // uniform; columns outside the grid are zeroed below
           const float4 nb = lnb(lp, c0 + k + R, lane);
           const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+          const int c = c0 + k;
+          if (lap_out && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
+            st4(lap_out + base + (y * P2 + x), lap);
           const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
           const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
           const float4 pc = cur[k];
@@ -962,6 +974,254 @@ __global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g
     k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lpa, lpb);
   else
     k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lpa, lpb);
+}
+
+// ------------------------------------------------------------------------------------------
+// temporally blocked reverse steps
+// ------------------------------------------------------------------------------------------
+// The reverse step has the forward step's shape: lambda^n from the stencil of
+// q = (m a) lambda^{n+1} and pointwise lambda^{n+1}, lambda^{n+2}. k5_tile advances a tile
+// through up to T reverse steps the way k4_tile advances p (same region, halo and LDS
+// exchange of the stencil operand; q instead of p), with two additions per level:
+// * the imaging sum a lambda^{n+1} * (Laplacian of p^n) over the output cells, from the
+//   Laplacians the forward sweep kept (pointwise), accumulated in float64 registers over the
+//   levels and the shot group and added to the group's gradient slice once;
+// * the residual injection of a recorded step: the level's lambda goes through LDS, the
+//   workgroup's threads add the precomputed sums at the injection cells inside the region
+//   (halo included, so the redundant cells stay exact), and the owners read it back.
+// The lambda values are bit-identical to single reverse steps followed by k_inject.
+constexpr int kAdjWaves = 8;  // wavefronts of a blocked reverse workgroup (256 registers each)
+
+template <int R, int ABS0, int TY, int T, bool Q>
+__device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& io, int shots,
+                                        int spw, const Tile& tl, float* lpa, float* lpb) {
+  constexpr int ABS = Q ? NW_NONE : ABS0;
+  using S = BlockShape<TY, R, T, kAdjWaves>;
+  constexpr int kInjChunks = kTapChunks * 256 / S::threads;
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ry0 = tl.ry * TY - S::hy;
+  const int rx0 = tl.cx * S::out_w - S::hx;
+  const int x = rx0 + 4 * lane;
+  const bool inx = Q || (x >= 0 && x < D2);
+  const bool outx = inx && lane >= S::hx / 4 && lane < 64 - S::hx / 4;
+  const int c0 = wave * S::mr;
+  const float4 hx = (ABS != NW_NONE && inx) ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<ABS> dcol = damp4_col<ABS>(hx);
+  float4 mm[S::mr];  // m a of this wavefront's rows (shot-independent)
+#pragma unroll
+  for (int k = 0; k < S::mr; ++k) {
+    const int y = ry0 + c0 + k;
+    mm[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.ma + (y * P2 + x)) : f4(0.f);
+  }
+  double2 ga[S::mr], gb[S::mr];  // float64 imaging sums over levels and shots
+#pragma unroll
+  for (int k = 0; k < S::mr; ++k) {
+    ga[k] = make_double2(0.0, 0.0);
+    gb[k] = ga[k];
+  }
+  auto load_lam2 = [&](int s, float4* dst) {
+    const int64_t base = (int64_t)s * g.cells;
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k) {
+      const int y = ry0 + c0 + k;
+      dst[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+    }
+  };
+  const int s_first = tl.grp * spw;
+  const int s_end = min(shots, s_first + spw);
+  float4 pvn[Q ? S::mr : 1];
+  if constexpr (Q) {
+    region_async<S, R>(io.lam1 + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
+    load_lam2(s_first, pvn);
+  }
+  auto shot = [&](auto bsel, int s) {
+    constexpr int B = decltype(bsel)::value;
+    const int64_t base = (int64_t)s * g.cells;
+    float* lp = (Q && B) ? lpb : lpa;
+    float4 pv[S::mr], cur[S::mr];  // lambda^{n+2}, lambda^{n+1} of the current level
+    if constexpr (Q) {
+      __builtin_amdgcn_s_waitcnt(0);  // this shot's region and lambda^{nh+1} rows have landed
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) pv[k] = pvn[k];
+    } else {
+      float4 v[S::per_thread];
+      const float* L1 = io.lam1 + base;
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) {
+        const int i = threadIdx.x + S::threads * k;
+        const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+        const int gy = ry0 + row, gx = rx0 - 4 + 4 * c4;
+        const bool in = (S::threads * (k + 1) <= S::n4 || i < S::n4);
+        const bool ok = in && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
+        v[k] = ok ? ld4(L1 + (gy * P2 + gx)) : f4(0.f);
+      }
+      load_lam2(s, pv);
+      __syncthreads();  // the previous shot's reads of lp are done
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) {
+        const int i = threadIdx.x + S::threads * k;
+        if (S::threads * (k + 1) <= S::n4 || i < S::n4)
+          *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
+      }
+    }
+    // injection cells of this shot inside the region: LDS offset (or -1) and value index
+    int ioff[kInjChunks];
+    bool any_inj = false;
+    const int n_inj = io.injv ? io.inj_count[s] : 0;
+    const int inj0 = io.injv ? io.inj_first[s] : 0;
+#pragma unroll
+    for (int c = 0; c < kInjChunks; ++c) {
+      const int u = threadIdx.x + S::threads * c;
+      ioff[c] = -1;
+      if (u < n_inj) {
+        const int idx = io.inj_cell[inj0 + u];
+        const int ty = idx / P2, tx = idx - ty * P2;
+        if (ty >= ry0 && ty < ry0 + S::crow && tx >= rx0 && tx < rx0 + 256) {
+          ioff[c] = (R + ty - ry0) * kLW + 4 + (tx - rx0);
+          any_inj = true;
+        }
+      }
+    }
+    if constexpr (!Q) __syncthreads();
+    if constexpr (Q) {
+      if (s + 1 < s_end) {  // stream the next shot in while this one computes
+        region_async<S, R>(io.lam1 + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
+        load_lam2(s + 1, pvn);
+      }
+    }
+    // lambda^{nh} of this wavefront's rows, then the stencil operand q = (m a) lambda in place
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+    const bool inj_tile = __syncthreads_or(any_inj);  // (also: every read of lambda is done)
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k)
+      *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = mul4(mm[k], cur[k]);
+    lds_barrier();
+    for (int lvl = 1; lvl <= io.levels; ++lvl) {
+      const int nk = io.nh - lvl;
+      const int tpn = (nk + 1) > io.first_step;
+      const float* lapl = io.lap[lvl];
+      float4 w[2 * R + 1];
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) w[j] = lrow(lp, c0 + j, lane);
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const int y = ry0 + c0 + k;
+        w[2 * R] = lrow(lp, c0 + k + 2 * R, lane);
+        float4 o = f4(0.f);
+        if (Q || (y >= 0 && y < D1)) {
+          const float4 nb = lnb(lp, c0 + k + R, lane);
+          const float4 lapq = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+          const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+          const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
+          const float4 lc = cur[k], l2c = pv[k];
+          float4 rb;
+          o = make_float4(adj1<ABS>(lc.x, l2c.x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
+                          adj1<ABS>(lc.y, l2c.y, lapq.y, dm.a.y, dm.b.y, dm.tap.y, tpn, rb.y),
+                          adj1<ABS>(lc.z, l2c.z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
+                          adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
+          if (!Q && !inx) o = f4(0.f);
+          const int c = c0 + k;
+          if (lapl && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1)) {
+            const float4 lapp = ld4(lapl + base + (y * P2 + x));
+            ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
+            ga[k].y = __builtin_fma((double)rb.y, (double)lapp.y, ga[k].y);
+            gb[k].x = __builtin_fma((double)rb.z, (double)lapp.z, gb[k].x);
+            gb[k].y = __builtin_fma((double)rb.w, (double)lapp.w, gb[k].y);
+          }
+        }
+        pv[k] = o;  // lambda^{n+2} is consumed: the slot takes lambda^n
+#pragma unroll
+        for (int j = 0; j < 2 * R; ++j) w[j] = w[j + 1];
+      }
+      const bool inject = inj_tile && io.inj_rec[lvl];
+      if (inject || lvl < io.levels) lds_barrier();  // every read of q^{n+1} is done
+      if (inject) {  // lambda^n += the residual sums, through LDS
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k)
+          *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = pv[k];
+        lds_barrier();
+        const float* iv = io.injv + ((int64_t)(lvl - 1) * shots + s) * io.inj_max;
+#pragma unroll
+        for (int c = 0; c < kInjChunks; ++c)
+          if (ioff[c] >= 0) lp[ioff[c]] = lp[ioff[c]] + iv[threadIdx.x + S::threads * c];
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k) pv[k] = lrow(lp, R + c0 + k, lane);
+        lds_barrier();
+      }
+      if (lvl < io.levels) {
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k)
+          *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) =
+              mul4(mm[k], pv[k]);
+        lds_barrier();
+      }
+      float* out = io.out[lvl];
+      if (out) {
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k) {
+          const int c = c0 + k, y = ry0 + c;
+          if (outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
+            st4(out + base + (y * P2 + x), pv[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const float4 t = cur[k];
+        cur[k] = pv[k];
+        pv[k] = t;
+      }
+    }
+  };
+  for (int s = s_first; s < s_end; s += 2) {
+    shot(std::integral_constant<int, 0>{}, s);
+    if (s + 1 < s_end) shot(std::integral_constant<int, 1>{}, s + 1);
+  }
+  if (io.grad) {
+    double* G = io.grad + (int64_t)tl.grp * g.cells;
+#pragma unroll
+    for (int k = 0; k < S::mr; ++k) {
+      const int c = c0 + k, y = ry0 + c;
+      if (outx && c >= S::hy && c < S::hy + TY && (Q || y < D1)) {
+        double2* gp = reinterpret_cast<double2*>(G + (y * P2 + x));
+        double2 a = gp[0], b = gp[1];
+        a.x += ga[k].x;
+        a.y += ga[k].y;
+        b.x += gb[k].x;
+        b.y += gb[k].y;
+        gp[0] = a;
+        gp[1] = b;
+      }
+    }
+  }
+}
+
+template <int R, int ABS, int TY, int T>
+__global__ void __launch_bounds__(64 * kAdjWaves) k5_adjoint(Geo<float> g, AdjBlockIO io,
+                                                             int shots, int spw, int xcd_map) {
+  using S = BlockShape<TY, R, T, kAdjWaves>;
+  __shared__ __align__(16) float lpa[S::lrows * kLW];
+  __shared__ __align__(16) float lpb[S::lrows * kLW];
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
+  const Tile tl = tile_of(div_up(D2, S::out_w), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
+    lpa[i] = 0.f;
+    lpa[(S::lrows - R) * kLW + i] = 0.f;
+    lpb[i] = 0.f;
+    lpb[(S::lrows - R) * kLW + i] = 0.f;
+  }
+  const int ry0 = tl.ry * TY - S::hy, rx0 = tl.cx * S::out_w - S::hx;
+  const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
+                     ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&
+                     rx0 >= g.quiet[2][0] && rx0 + 256 <= g.quiet[2][1];
+  if (quiet)
+    k5_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lpa, lpb);
+  else
+    k5_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lpa, lpb);
 }
 
 // trace samples of the intermediate levels of a blocked launch, from its tap buffer: the
@@ -1088,6 +1348,10 @@ bool fast2d_block_supported(int ty, int depth) {
   return (ty == 8 || ty == 16 || ty == 32) && depth >= 2 && depth <= kMaxDepth;
 }
 
+bool fast2d_adj_block_supported(int ty, int depth) {
+  return (ty == 8 || ty == 16) && depth >= 2 && depth <= kMaxDepth;
+}
+
 template <int R, int ABS, int TY, int T>
 static void launch_steps_t(const Geo<float>& g, const BlockIO& io, int shots, int spw, int xm,
                            hipStream_t st) {
@@ -1129,6 +1393,50 @@ void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& 
     case NW_SPONGE * 2 + 1: steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
     case NW_CERJAN * 2 + 0: steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
     default: steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+  }
+}
+
+template <int R, int ABS, int TY, int T>
+static void launch_adj_steps_t(const Geo<float>& g, const AdjBlockIO& io, int shots, int spw,
+                               int xm, hipStream_t st) {
+  using S = BlockShape<TY, R, T, kAdjWaves>;
+  const int tiles =
+      div_up((int)g.D[2], S::out_w) * div_up((int)g.D[1], TY) * div_up(shots, spw);
+  hipLaunchKernelGGL((k5_adjoint<R, ABS, TY, T>),
+                     dim3((unsigned)(kXcds * div_up(tiles, kXcds))), dim3(S::threads), 0, st, g,
+                     io, shots, spw, xm & 1);
+}
+
+template <int R, int ABS, int TY>
+static void adj_steps_depth(const Geo<float>& g, const AdjBlockIO& io, int shots, int depth,
+                            int spw, int xm, hipStream_t st) {
+  switch (depth) {
+    case 2: launch_adj_steps_t<R, ABS, TY, 2>(g, io, shots, spw, xm, st); break;
+    case 3: launch_adj_steps_t<R, ABS, TY, 3>(g, io, shots, spw, xm, st); break;
+    default: launch_adj_steps_t<R, ABS, TY, 4>(g, io, shots, spw, xm, st); break;
+  }
+}
+
+template <int R, int ABS>
+static void adj_steps_ty(const Geo<float>& g, const AdjBlockIO& io, int shots, int ty,
+                         int depth, int spw, int xm, hipStream_t st) {
+  if (ty == 8)
+    adj_steps_depth<R, ABS, 8>(g, io, shots, depth, spw, xm, st);
+  else
+    adj_steps_depth<R, ABS, 16>(g, io, shots, depth, spw, xm, st);
+}
+
+void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const AdjBlockIO& io,
+                          int shots, int ty, int depth, int spw, int xm, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int key = absorbing * 2 + (order == 4 ? 1 : 0);
+  switch (key) {
+    case NW_NONE * 2 + 0: adj_steps_ty<1, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_NONE * 2 + 1: adj_steps_ty<2, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_SPONGE * 2 + 0: adj_steps_ty<1, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_SPONGE * 2 + 1: adj_steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_CERJAN * 2 + 0: adj_steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    default: adj_steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
   }
 }
 

@@ -53,6 +53,7 @@ struct StepIO {
   T* ck_prev;          // checkpoint slot (or null)
   T* ck_cur;
   T* ck_psi[3];
+  T* lap_out;          // receives the Laplacian of p^n (fast 2-D kernels; or null)
 };
 
 template <typename T>
@@ -89,6 +90,7 @@ struct AdjIO {
   T* lam0;             // lam^n (may alias lam2: read pointwise before the write)
   int taper_next;      // Cerjan: step n+1 tapers its p^{n}
   const T* p_n;        // p^n for the imaging term (or null)
+  const T* lap_n;      // or the Laplacian of p^n itself, kept by the forward sweep (fast 2-D)
   const T* psi_low[3];   // psi^{n-1/2} (PML imaging)
   const T* psi_half[3];  // psi^{n+1/2}
   const T* psib_in[3];   // adjoint of psi^{n+1/2}
@@ -106,6 +108,7 @@ struct BlockIO {
   const float* p_prev;       // p^{n0-1}
   const float* p;            // p^{n0}
   float* out[kMaxDepth + 1];
+  float* lap_out[kMaxDepth + 1];  // level k: the Laplacian of p^{n0+k-1} (or null)
   int levels;                // 1 .. depth of the kernel
   int n0;
   int first_step;            // Cerjan: step n tapers its p^{n-1} when n > first_step
@@ -126,6 +129,28 @@ struct BlockIO {
   const int32_t* rec_idx;
 };
 
+// Temporally blocked reverse steps (2-D float32): levels k = 1..levels compute lambda^{nh-k}
+// from lambda^{nh} (lam1) and lambda^{nh+1} (lam2).
+struct AdjBlockIO {
+  const float* ma;                  // m * a_eff
+  const float* lam1;
+  const float* lam2;
+  float* out[kMaxDepth + 1];        // level k receives lambda^{nh-k} (or null)
+  const float* lap[kMaxDepth + 1];  // level k: Laplacian of p^{nh-k} for the imaging sum (or null)
+  double* grad;                     // one float64 accumulator grid per shot group (or null)
+  int levels;
+  int nh;
+  int first_step;                   // Cerjan: step n tapers its p^{n-1} when n > first_step
+  // residual injection after level k when inj_rec[k]: injv[((k-1) * n_shots + s) * inj_max + u]
+  // is added at cell inj_cell[inj_first[s] + u] (the sums k_inject adds)
+  const float* injv;
+  int inj_max;
+  int inj_rec[kMaxDepth + 1];
+  const int32_t* inj_first;
+  const int32_t* inj_count;
+  const int32_t* inj_cell;
+};
+
 // fast 2-D float32 kernels (nefi_wave_2d.hip)
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype);
 void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
@@ -140,6 +165,9 @@ void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& 
 struct LevelRecords {
   int j[kMaxDepth];
 };
+bool fast2d_adj_block_supported(int ty, int depth);
+void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const AdjBlockIO& io,
+                          int shots, int ty, int depth, int spw, int xcd_map, void* stream);
 void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const float* tapbuf,
                           int tap_stride, int rec_max, int shots, int levels,
                           const LevelRecords& lj, void* stream);

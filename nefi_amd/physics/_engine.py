@@ -442,7 +442,8 @@ class Propagator:
                 state[1].copy_(cur)
                 for a, f in enumerate(faces):
                     psi[a].copy_(f)
-                n_end = n1 if P.pml else max(n0, n1 - 1)
+                # keep slots hold p^n, or the Laplacian of p^n, which step n itself computes
+                n_end = n1 if P.pml or P.option(nat.OPT_KEEP_LAP) else max(n0, n1 - 1)
                 P.forward(m, state, psi, n0, n_end, keep=keep)
             P.adjoint(ma, lam, psib, n0, n1, keep=keep, residual=res,
                       inject_end=(b == nb - 1), grad=grad_m)

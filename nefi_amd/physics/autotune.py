@@ -29,7 +29,8 @@ from ..errors import NativeError
 ROWS = (4, 8, 16, 32, 64)
 LDS_ROWS = (8, 16, 32)
 GROUPS = (1, 2, 4, 8, 16)
-DEPTHS = (2, 3, 4)  # temporal blocking depths of the forward step
+DEPTHS = (2, 3, 4)  # temporal blocking depths
+ADJ_BLOCK_ROWS = (8, 16)  # reverse tiles with a blocked kernel
 BLOCK_STEPS = 12  # forward steps per timing: a multiple of every depth
 # below this many cell-updates per step a launch is latency-bound and every tile ties
 MIN_WORK = 1 << 18
@@ -67,7 +68,7 @@ def plan_key(plan) -> str:
         name = f"{getattr(props, 'gcnArchName', '') or props.name}/{props.multi_processor_count}"
     else:
         name = "cpu"
-    return f"v2|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
+    return f"v3|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
 
 
 def candidates(kind: int, n_shots: int):
@@ -78,6 +79,8 @@ def candidates(kind: int, n_shots: int):
     out += [(r, g, 1, 1, 1) for r in LDS_ROWS for g in groups]
     if kind == 0:
         out += [(r, g, 1, 1, t) for r in LDS_ROWS for g in groups for t in DEPTHS]
+    else:
+        out += [(r, g, 1, 1, t) for r in ADJ_BLOCK_ROWS for g in groups for t in DEPTHS]
     return out
 
 
@@ -86,6 +89,8 @@ def _apply(plan, kind: int, rows: int, spw: int, rpi: int, lds: int, depth: int 
     if kind == 0:
         plan.set_option(nat.OPT_ROWS_PER_ITER, rpi)
         plan.set_option(nat.OPT_TIME_BLOCK, depth)
+    else:
+        plan.set_option(nat.OPT_TIME_BLOCK_ADJOINT, depth)
     plan.set_option(nat.OPT_LDS_TILES_STEP if kind == 0 else nat.OPT_LDS_TILES_ADJOINT, lds)
 
 
@@ -107,8 +112,11 @@ def _time_kind(plan, kind: int, cand, steps: int) -> float:
         keep = P.zeros((steps + 1) * P.keep_slot)
         lam = [P.zeros(P.shot_numel), P.zeros(P.shot_numel)]
         grad = P.zeros(P.grad_slices() * cells, torch.float64)
-        P.adjoint(m, lam, psi, 0, steps, keep=keep, grad=grad)
-        ms, n = nat.profile_read(nat.K_ADJOINT)
+        res = P.zeros(max(1, P.host.n_trace_rows * P.host.n_t))  # residual injection included
+        P.adjoint(m, lam, psi, 0, steps, keep=keep, grad=grad, residual=res)
+        kinds = (nat.K_ADJOINT, nat.K_ADJOINT_STEPS)
+        ms = sum(nat.profile_read(k)[0] for k in kinds) + nat.profile_read(nat.K_INJECT)[0]
+        n = sum(nat.profile_units(k) for k in kinds)
     nat.profile_enable(False)
     return ms / max(1, n)
 
@@ -134,7 +142,7 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
             pick = {"xcd_map": xcd, "table": {}}
             for kind, name in ((0, "step"), (1, "adjoint")):
                 timings = {}
-                n = steps if kind == 1 else min(BLOCK_STEPS, max(1, h.n_steps))
+                n = min(BLOCK_STEPS, max(1, h.n_steps))
                 for cand in candidates(kind, h.n_shots):
                     try:
                         _time_kind(plan, kind, cand, 1)  # warm the candidate once

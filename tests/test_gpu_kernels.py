@@ -40,7 +40,7 @@ def _survey(absorbing="sponge", order=4, n=(70, 300), shots=5, n_t=60):
     return Domain(n, ext), geo, c, kw
 
 
-def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1):
+def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1, keep_lap=1, depth_adj=1):
     from nefi_amd import _native as nat
 
     x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
@@ -54,6 +54,8 @@ def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1):
         prop.plan.set_option(nat.OPT_LDS_TILES_ADJOINT, lds[1])
         prop.plan.set_option(nat.OPT_XCD_MAP, xcd)
         prop.plan.set_option(nat.OPT_TIME_BLOCK, depth)
+        prop.plan.set_option(nat.OPT_KEEP_LAP, keep_lap)
+        prop.plan.set_option(nat.OPT_TIME_BLOCK_ADJOINT, depth_adj)
         y = op({"c": x})
     (g,) = torch.autograd.grad((y * w).sum(), x)
     return y.detach().cpu().double().numpy(), g.cpu().double().numpy()
@@ -109,6 +111,92 @@ def test_time_blocking_is_bit_identical(absorbing, order):
               f"{blocked} blocked steps, traces {ey:.1e}, gradient {eg:.1e}", flush=True)
         assert blocked > 0
         assert np.array_equal(y, y0) and np.array_equal(g, g0)
+
+
+@pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2)])
+def test_kept_laplacian_matches_kept_wavefield(absorbing, order):
+    """Keep buffers holding the Laplacian of p^n (written by the forward kernels, read pointwise
+    by the reverse step) give exactly the gradient of keep buffers holding p^n (the reverse step
+    applying the stencil itself): register march and LDS kernels, blocked and one-step sweeps."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey(absorbing, order, n=(70, 600))
+    print(f"[case kept-laplacian {absorbing} o{order}]", flush=True)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(7))
+    for tiles, lds, depth in (([(16, 2), (16, 2)], (1, 1), 1), ([(8, 1), (8, 3)], (0, 0), 1),
+                              ([(32, 4), (8, 2)], (1, 1), 4), ([(16, 2), (16, 5)], (1, 0), 2)):
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=lds, depth=depth, keep_lap=0)
+        y, g = _grad(op, c, w, tiles=tiles, lds=lds, depth=depth, keep_lap=1)
+        plan = next(iter(op._cache.values())).plan
+        used = plan.option(nat.OPT_KEEP_LAP)
+        print(f"{absorbing} o{order} tiles {tiles} lds {lds} depth {depth}: kept laplacian "
+              f"{used}, traces {rel(y, y0):.1e}, gradient {rel(g, g0):.1e}", flush=True)
+        assert used == 1
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
+
+
+@pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2),
+                                             ("sponge", 2)])
+def test_blocked_reverse_matches_single_steps(absorbing, order):
+    """Temporally blocked reverse launches (depth 2-4: residual injection at intermediate
+    levels, imaging from kept Laplacians, quiet and boundary tiles) give the gradient of one
+    reverse launch per step; only the float64 imaging sums are added in another order."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey(absorbing, order, n=(70, 600))
+    print(f"[case blocked-reverse {absorbing} o{order}]", flush=True)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(8))
+    for tiles, depth, depth_adj in (([(16, 2), (16, 2)], 1, 2), ([(32, 4), (8, 3)], 4, 3),
+                                    ([(16, 1), (16, 5)], 2, 4), ([(8, 2), (8, 1)], 3, 4)):
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=(1, 1), depth=depth, depth_adj=1)
+        nat.profile_enable(True)
+        y, g = _grad(op, c, w, tiles=tiles, lds=(1, 1), depth=depth, depth_adj=depth_adj)
+        blocked = nat.profile_units(nat.K_ADJOINT_STEPS)
+        nat.profile_enable(False)
+        eg = rel(g, g0)
+        print(f"{absorbing} o{order} tiles {tiles} depth {depth}/{depth_adj}: {blocked} blocked "
+              f"reverse steps, traces {rel(y, y0):.1e}, gradient {eg:.1e}", flush=True)
+        assert blocked > 0
+        assert np.array_equal(y, y0) and eg < 1e-6
+
+
+def test_blocked_reverse_initial_condition_2d():
+    """The initial-value operator's reverse sweep (residual injected at every recorded step, no
+    imaging sum) is bit-identical blocked and stepped one by one."""
+    from nefi_amd import _native as nat
+    from nefi_amd.physics.wave import WaveInitialConditionOperator
+
+    n, h = (64, 320), 10.0
+    dom = Domain(n, ((0.0, n[0] * h), (0.0, n[1] * h)))
+    rec = [[h * 2.5, h * (4 + 3 * i)] for i in range(90)]
+    op = WaveInitialConditionOperator(dom, rec, n_t=70, dt_obs=0.002, c=1600.0, order=4,
+                                      absorbing="sponge", absorb_width=80.0)
+    rng = np.random.default_rng(9)
+    p0 = torch.tensor(rng.standard_normal(n), dtype=torch.float32, device=DEV,
+                      requires_grad=True)
+    w = torch.randn(len(rec), 70, device=DEV, generator=torch.Generator(DEV).manual_seed(10))
+    grads = {}
+    for depth_adj in (1, 2, 3, 4):
+        op({"p0": p0})  # the plan exists after the first call
+        prop = next(iter(op._cache.values()))
+        prop.plan.set_tiles(1, 16, 1)
+        prop.plan.set_option(nat.OPT_LDS_TILES_ADJOINT, 1)
+        prop.plan.set_option(nat.OPT_TIME_BLOCK_ADJOINT, depth_adj)
+        nat.profile_enable(True)
+        (g,) = torch.autograd.grad((op({"p0": p0}) * w).sum(), p0)
+        blocked = nat.profile_units(nat.K_ADJOINT_STEPS)
+        nat.profile_enable(False)
+        grads[depth_adj] = g.cpu().numpy()
+        print(f"IC 2-D reverse depth {depth_adj}: {blocked} blocked steps, gradient vs single "
+              f"steps {rel(grads[depth_adj], grads[1]):.1e}", flush=True)
+        assert (blocked > 0) == (depth_adj > 1)
+        assert np.array_equal(grads[depth_adj], grads[1])
 
 
 def test_fast_path_against_oracle_uneven_shots():
