@@ -183,11 +183,12 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   8, 16 or 32 and, for calls that record traces, a shot has at most 2048 receiver taps;
  *   other calls step one by one. Setting T > 1 allocates two wavefields and a small tap
  *   buffer held by the plan.
- * NW_OPT_KEEP_LAP: 1 (default) lets keep buffers hold the Laplacian of p^n instead of p^n
- *   whenever both kernel kinds run the fast 2-D kernels (reads back as 1 only then): the
- *   forward kernels write it as they step, and the reverse kernels read it pointwise for the
- *   imaging term (same values, no stencil). A keep sweep feeding such a reverse sweep over
- *   steps [n0, n1) must then run forward over [n0, n1) (not n1 - 1). 0 keeps p^n.
+ * NW_OPT_KEEP_LAP: 1 lets keep buffers hold the Laplacian of p^n instead of p^n whenever both
+ *   kernel kinds run the fast 2-D kernels (reads back as 1 only then): the forward kernels
+ *   write it as they step, and the reverse kernels read it pointwise for the imaging term
+ *   (same values, no stencil). A keep sweep feeding such a reverse sweep over steps [n0, n1)
+ *   must then run forward over [n0, n1) (not n1 - 1). 0 (default) keeps p^n: a blocked keep
+ *   sweep then writes only the levels themselves, which measured faster overall.
  * NW_OPT_TIME_BLOCK_ADJOINT: temporal blocking depth of the fast reverse step (1 default,
  *   2..4): up to that many reverse steps per launch, with the residual injections of recorded
  *   steps and the imaging sum inside the launch (lambda bit-identical to single steps; the
