@@ -169,25 +169,48 @@ def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
     return grad
 
 
-def autograd_comparator(op, c, residual, shots=8, steps=200):
-    """Time the autograd path on `shots` shots x `steps` steps, with every step's tape kept
-    (grad_mode="autograd") and with checkpoint blocks of 200 (grad_mode="checkpoint"), and
-    scale linearly to the workload (a whole-workload autograd run needs terabytes)."""
+def autograd_comparator(op, c, residual, steps=200):
+    """Time the plain autograd path with every step's tape kept (grad_mode="autograd") and
+    with checkpoint blocks of 200 (grad_mode="checkpoint"), each at the source batch that fits
+    HBM for the whole workload, and scale linearly to the workload (one batch of the
+    store-all path's tape over all 4000 steps would not fit, so `steps` steps are timed).
+
+    Batch sizing: the tape bytes per shot-step are measured on a short probe; store-all keeps
+    n_steps of them per shot, checkpointing keeps one block of 200 plus two wavefields per
+    snapshot. The budget is 85 % of the free device memory."""
     import torch
 
-    scale = (op.n_steps / steps) * (len(op.shot_sources) / shots)
+    n_shots = len(op.shot_sources)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    probe = 40
+    autograd_leapfrog(op, c, 1, probe, residual, block=None)  # also warms MIOpen, allocator
+    torch.cuda.synchronize()
+    tape = (torch.cuda.max_memory_allocated() - base) / probe
+    torch.cuda.empty_cache()
+    budget = 0.85 * torch.cuda.mem_get_info()[0]
+    state = op.grid.padded_shape[0] * op.grid.padded_shape[1] * c.element_size()
+    per_shot = {"autograd": tape * op.n_steps,
+                "checkpoint": tape * 200 + 2 * state * -(-op.n_steps // 200)}
     out = {}
     for mode, block in (("autograd", None), ("checkpoint", 200)):
-        autograd_leapfrog(op, c, shots, 20, residual, block=block)  # warm up MIOpen, allocator
+        batch = int(max(1, min(n_shots, budget // per_shot[mode])))
+        autograd_leapfrog(op, c, batch, 20, residual, block=block)  # warm up this batch
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        autograd_leapfrog(op, c, shots, steps, residual, block=block)
+        autograd_leapfrog(op, c, batch, steps, residual, block=block)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
+        scale = (op.n_steps / steps) * (n_shots / batch)
         out[mode] = {
             "ms_per_gradient_scaled": round(t * scale * 1e3, 1),
-            "sample": f"{shots} shots x {steps} steps measured {t:.2f} s, scaled x{scale:.0f} "
-                      f"to {len(op.shot_sources)} shots x {op.n_steps} steps",
+            "source_batch": batch,
+            "sample": f"source_batch {batch} (tape {tape / 2**20:.1f} MiB per shot-step, "
+                      f"{per_shot[mode] / 2**30:.1f} GiB per shot, budget "
+                      f"{budget / 2**30:.0f} GiB): {batch} shots x {steps} steps measured "
+                      f"{t:.2f} s, scaled x{scale:.0f} to {n_shots} shots x {op.n_steps} steps",
         }
         torch.cuda.empty_cache()
     return out
@@ -381,6 +404,8 @@ def main() -> int:
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
     comp = None
     if not args.no_autograd:
+        op._cache.clear()  # release the HIP path's workspace: the comparator gets the device
+        torch.cuda.empty_cache()
         comp = autograd_comparator(op, c, residual)
         for entry in comp.values():
             entry["speedup"] = round(entry["ms_per_gradient_scaled"] / ms_per_step, 1)

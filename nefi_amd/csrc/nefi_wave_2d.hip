@@ -691,12 +691,14 @@ __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io,
 // LDS at every intermediate level into a tap buffer, which a small kernel turns into trace
 // samples with k_sample's tap order; the last level is sampled from memory as usual.
 // wavefronts of one blocked workgroup: 16 for short tiles (more wavefronts per SIMD to hide
-// LDS latency), 8 for 32-row tiles (whose per-lane rows would not fit 128 registers)
+// LDS latency), 12 for 32-row tiles (4 of the 48 region rows each at depth 4, within the 170
+// registers of three wavefronts per SIMD)
 template <int TY>
 struct BlockWaves {
-  static constexpr int waves = TY >= 32 ? 8 : 16;
+  static constexpr int waves = TY >= 32 ? 12 : 16;
   static constexpr int threads = 64 * waves;
-  static constexpr int tap_chunks = kTapChunks * 256 / threads;
+  // receiver taps one workgroup scans per shot: at least 256 * kTapChunks
+  static constexpr int tap_chunks = (kTapChunks * 256 + threads - 1) / threads;
 };
 
 template <int TY, int R, int T, int W = BlockWaves<TY>::waves>
@@ -882,7 +884,10 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
         const int y = ry0 + c0 + k;
         w[2 * R] = lrow(lp, c0 + k + 2 * R, lane);
         float4 o = f4(0.f);
-        if (Q || (y >= 0 && y < D1)) {  // uniform; columns outside the grid are zeroed below
+        // rows that can still reach the output tile: [R lvl, rows - R lvl) (uniform); the rim
+        // outside keeps stale values, which no later level reads
+        const bool live = c0 + k >= R * lvl && c0 + k < S::rows - R * lvl;
+        if (live && (Q || (y >= 0 && y < D1))) {  // columns outside the grid are zeroed below
           const float4 nb = lnb(lp, c0 + k + R, lane);
           const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
           const int c = c0 + k;

@@ -113,6 +113,35 @@ def test_time_blocking_is_bit_identical(absorbing, order):
         assert np.array_equal(y, y0) and np.array_equal(g, g0)
 
 
+def test_time_blocking_with_2048_receiver_taps_per_shot():
+    """Shots at the tap limit of a blocked launch (512 receivers x 4 taps = 2048, as in the
+    64-shot benchmark survey) sample every intermediate level exactly."""
+    from nefi_amd import _native as nat
+
+    n, h = (48, 600), 10.0
+    dom = Domain(n, ((0.0, n[0] * h), (0.0, n[1] * h)))
+    geo = []
+    for s in range(3):
+        src = [[(s + 1) * 0.25 * n[0] * h, 120.0 + 10.0 * s]]
+        r = (512, 300, 511)[s]
+        rec = [[(2.0 + 0.3 * (i % 3)) * h, (1.0 + (n[1] - 2.0) * (i + 0.5) / r) * h]
+               for i in range(r)]
+        geo.append((src, rec))
+    c = 1500.0 + 200.0 * np.random.default_rng(11).random(n)
+    kw = dict(n_t=40, dt_obs=0.002, f0=15.0, c_max=1700.0, order=4, absorbing="sponge",
+              absorb_width=80.0, grad_mode="checkpoint", checkpoint_every=13)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, generator=torch.Generator(DEV).manual_seed(12))
+    for ty, depth in ((32, 4), (16, 3), (8, 2), (32, 2)):
+        y0, g0 = _grad(op, c, w, tiles=[(ty, 2), (16, 2)], lds=(1, 1), depth=1)
+        y, g = _grad(op, c, w, tiles=[(ty, 2), (16, 2)], lds=(1, 1), depth=depth)
+        print(f"2048 taps rows {ty} depth {depth}: traces {rel(y, y0):.1e}, "
+              f"gradient {rel(g, g0):.1e}", flush=True)
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
+    assert nat.OPT_TIME_BLOCK == 6
+
+
 @pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2)])
 def test_kept_laplacian_matches_kept_wavefield(absorbing, order):
     """Keep buffers holding the Laplacian of p^n (written by the forward kernels, read pointwise
