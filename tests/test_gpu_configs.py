@@ -5,8 +5,9 @@
   differences along 16 seeded unit directions, and the float32 gradient (fast kernels) against
   the float64 one.
 * Config 4 (3-D photoacoustic, 192³ + 10-cell sponge = 212³, 4096 detectors, 1500 steps): the
-  adjoint dot-product test ⟨A p0, w⟩ = ⟨p0, Aᵀ w⟩ of the initial-value operator, and float32
-  traces against float64.
+  adjoint dot-product test ⟨A p0, w⟩ = ⟨p0, Aᵀ w⟩ of the initial-value operator, float32
+  traces against float64, and the float64 p0 gradient of a quadratic misfit against central
+  differences along 4 seeded directions.
 * Config 5 (512² + 20-cell sponge = 552², 32 shots, 3000 steps, layered model): the float32
   gradient (fast kernels, autotuned) against the float64 gradient, within the stated 1e-4.
 
@@ -122,6 +123,28 @@ def test_config4_pat3d_adjoint_dot_product_and_float32_traces():
     e = rel(y.detach().double().cpu().numpy(), y64.cpu().numpy())
     print(f"float32 traces vs float64: {e:.2e}", flush=True)
     assert e < 1e-5
+    # float64 gradient of J = 1/2 |A p0 - d|^2 against central differences along 4 seeded unit
+    # directions; J is quadratic in p0, so the central difference is exact up to round-off
+    d = (y64 + 0.1 * w.double() * y64.abs().max()).detach()
+    x64 = x.detach().double().requires_grad_(True)
+
+    def J(p):
+        return 0.5 * ((op({"p0": p}) - d) ** 2).sum()
+
+    (g64,) = torch.autograd.grad(J(x64), x64)
+    eps = 1e-2 * float(x64.detach().norm())
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(4):
+            e_k = torch.randn(x64.shape, device=DEV, dtype=torch.float64,
+                              generator=torch.Generator(DEV).manual_seed(20 + k))
+            e_k /= e_k.norm()
+            fd = (J(x64 + eps * e_k) - J(x64 - eps * e_k)).item() / (2 * eps)
+            ad = float((g64 * e_k).sum())
+            worst = max(worst, abs(fd - ad) / abs(fd))
+            print(f"p0 direction {k}: FD {fd:.9e} adjoint {ad:.9e}", flush=True)
+    print(f"float64 p0 gradient vs central differences: {worst:.1e}", flush=True)
+    assert worst < 1e-8
 
 
 def test_config5_float32_gradient_against_float64():
