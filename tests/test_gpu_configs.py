@@ -148,25 +148,10 @@ def test_config4_pat3d_adjoint_dot_product_and_float32_traces():
 
 
 def test_config5_float32_gradient_against_float64():
-    n, dx = 512, 10.0
-    L = n * dx
-    rng = np.random.default_rng(4)
-    xs = (np.arange(n) + 0.5) * dx
-    zs = (np.arange(n) + 0.5) * dx
-    # six layers with undulating interfaces (20 cells) and two Gaussian inclusions
-    speeds = [1600.0, 2000.0, 2500.0, 3000.0, 3600.0, 4200.0]
-    c = np.full((n, n), speeds[0])
-    for k in range(1, 6):
-        depth = k * L / 6 + 20 * dx * np.sin(2 * np.pi * xs / L * (k + 1) + rng.uniform(0, 6))
-        c[zs[None, :] > depth[:, None]] = speeds[k]
-    for cx, cz, a in ((0.3 * L, 0.4 * L, 400.0), (0.7 * L, 0.65 * L, -300.0)):
-        c += a * np.exp(-((xs[:, None] - cx) ** 2 + (zs[None, :] - cz) ** 2) / (2 * (25 * dx) ** 2))
-    c = np.clip(c, 1500.0, 4500.0)
-    src = [[(i + 0.5) * L / 32, 0.5 * dx] for i in range(32)]
-    rec = [[(i + 0.5) * L / 256, 0.5 * dx] for i in range(256)]
-    op = WaveOperator(Domain((n, n), ((0.0, L), (0.0, L))), src, rec, n_t=3001,
-                      dt_obs=0.4 * dx / 4500.0, f0=15.0, c_max=4500.0, order=4,
-                      absorbing="sponge", absorb_width=20 * dx, grad_mode="checkpoint")
+    from nefi_amd.drivers.neural_fwi import layered_model, survey
+
+    c = layered_model(512)
+    op = survey(512, 32, 256, 3001)
     print(f"[case config 5: padded {op.grid.padded_shape}, {op.n_steps} steps, 32 shots]",
           flush=True)
     res = torch.randn(32, 256, 3001, device=DEV, dtype=torch.float64,
