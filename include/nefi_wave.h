@@ -16,7 +16,8 @@
  *    Wavefield rows are `pitch` elements apart (see nw_desc.pitch); the sound speed c,
  *    initial pressure p0 and the outputs on the field grid are dense.
  *  - A wavefield buffer holds `n_shots` consecutive grids. PML auxiliary fields are face
- *    grids (extent + 1 along their own axis), also `n_shots` consecutive.
+ *    grids (extent + 1 along their own axis), also `n_shots` consecutive; a face buffer is
+ *    rounded up to a multiple of 64 elements, so every slot offset stays 256-byte aligned.
  *  - `dtype` selects float (0) or double (1) for every real-valued buffer of a plan.
  *  - Every call is stream-ordered on `stream` (a hipStream_t passed as void*; NULL = default
  *    stream) and does not synchronise the host, except `nw_plan_create`.

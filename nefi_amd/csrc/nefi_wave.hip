@@ -760,14 +760,16 @@ static int hand_back(nw_plan* P, void** state, T* s0, T* s1, T* prev, T* cur, hi
 // run: the forward kernels compute it anyway, and the reverse kernels' imaging term then reads
 // it pointwise instead of re-applying the stencil to a p tile.
 static bool keep_lap(const nw_plan* P) {
-  return P->d.dtype == NW_F32 && P->keep_lap_opt && P->rows[0] > 0 && P->rows[1] > 0;
+  return P->d.dtype == NW_F32 && P->d.absorbing != NW_PML && P->keep_lap_opt && P->rows[0] > 0 &&
+         P->rows[1] > 0;
 }
 
 // The blocked sweep applies when the plan asks for it and the forward tiles have a blocked
 // kernel; a sweep recording traces also needs every shot's receiver taps in one scan.
 static bool use_blocked(const nw_plan* P, bool traces) {
   const nw_desc& d = P->d;
-  return d.dtype == NW_F32 && P->tblock > 1 && P->rows[0] > 0 && P->tb_state[0] &&
+  return d.dtype == NW_F32 && d.absorbing != NW_PML && P->tblock > 1 && P->rows[0] > 0 &&
+         P->tb_state[0] &&
          fast2d_block_supported(P->rows[0], P->tblock) &&
          (!traces || d.rec_max * d.rec_taps <= 256 * kTapChunks);
 }
@@ -991,7 +993,8 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
 // fit one scan of the workgroup.
 static bool use_blocked_adj(const nw_plan* P, const void* keep, const void* residual) {
   const nw_desc& d = P->d;
-  return d.dtype == NW_F32 && P->tblock_adj > 1 && P->rows[1] > 0 && P->tb_state[0] &&
+  return d.dtype == NW_F32 && d.absorbing != NW_PML && P->tblock_adj > 1 && P->rows[1] > 0 &&
+         P->tb_state[0] &&
          P->injv && fast2d_adj_block_supported(P->rows[1], P->tblock_adj) &&
          (!keep || keep_lap(P)) && (!residual || d.inj_max <= 256 * kTapChunks);
 }
@@ -1324,7 +1327,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->shot_bytes = cells * (int64_t)es * d.n_shots;
   for (int a = 0; a < 3; ++a)
     p->face_bytes[a] = (d.absorbing == NW_PML && a >= 3 - d.ndim)
-                           ? p->gf.fcells[a] * (int64_t)es * d.n_shots
+                           ? (p->gf.fcells[a] * d.n_shots + 63) / 64 * 64 * (int64_t)es
                            : 0;
   *out = p;
   return NW_OK;
@@ -1410,8 +1413,10 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     if (value < 1 || value > kMaxDepth)
       return fail(NW_ERR_ARG, "time block depth must be 1.." + std::to_string(kMaxDepth));
     if (value > 1 && !P->tb_state[0]) {
-      if (P->d.dtype != NW_F32 || !fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
-        return fail(NW_ERR_UNSUPPORTED, "time blocking needs the fast 2-D float32 kernels");
+      if (P->d.dtype != NW_F32 || P->d.absorbing == NW_PML ||
+          !fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
+        return fail(NW_ERR_UNSUPPORTED,
+                    "time blocking needs the fast 2-D float32 kernels without a PML");
       const int64_t taps = (int64_t)(kMaxDepth - 1) * P->d.n_shots *
                            std::max(1, P->d.rec_max * P->d.rec_taps);
       const int64_t inj = (int64_t)kMaxDepth * P->d.n_shots * std::max(1, P->d.inj_max);

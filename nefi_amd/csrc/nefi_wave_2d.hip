@@ -79,7 +79,7 @@ __device__ __forceinline__ void damp1(float hy, float hx, float& a, float& b, fl
   a = 1.f;
   b = 1.f;
   tap = 1.f;
-  if (ABS == NW_SPONGE) {
+  if (ABS == NW_SPONGE || ABS == NW_PML) {
     const float half = hy + hx;
     a = 1.f / (1.f + half);
     b = 1.f - half;
@@ -200,7 +200,7 @@ __device__ __forceinline__ Damp4<ABS> damp4_col(const float4& hx) {
 template <int ABS>
 __device__ __forceinline__ Damp4<ABS> damp4_row(float hy, const float4& hx,
                                                 const Damp4<ABS>& col) {
-  if (ABS == NW_SPONGE && hy == 0.f) return col;
+  if ((ABS == NW_SPONGE || ABS == NW_PML) && hy == 0.f) return col;
   return damp4<ABS>(hy, hx);
 }
 
@@ -1252,10 +1252,359 @@ __global__ void __launch_bounds__(256) k_sample_levels(Geo<float> g, SampleIO<fl
 }
 
 // ------------------------------------------------------------------------------------------
+// PML (2-D): LDS-tiled forward and reverse steps with the face fields
+// ------------------------------------------------------------------------------------------
+// Face fields psi_a live on the faces normal to axis a: psi_y rows of D2 faces (D1 + 1 rows),
+// psi_x rows of D2 + 1 faces (D1 rows), unpitched. A lane's four cells x..x+3 of row y touch
+// x-faces x..x+4 (face x+4 is the next lane's face x: it computes it again, with the same
+// operations) and y-faces of rows y and y+1 (the next row of the wavefront computes row y+1
+// again). Each face is written by the cell on its high side; the last cell of a row (column)
+// also writes the final face. Per face f between cells f-1 and f:
+//   forward:  psi'_f = alpha_f psi_f + (zeta_other - zf_f) kappa_f (p_f - p_{f-1})
+//             lap += ((psi'_{f+1} + psi_{f+1}) - (psi'_f + psi_f)) / (2h)   (per cell)
+//   reverse:  t_f = psibar_f - (q_f - q_{f-1}) / (2h)
+//             lambda -= (zeta_other - zf_{f+1}) kappa_{f+1} t_{f+1} - (...)_f t_f
+//             psibar'_f = alpha_f t_f - (q_f - q_{f-1}) / (2h)
+// following nefi/physics/wave.py:529-552 and its adjoint, in the generic kernels' order
+// (Laplacian, then the y faces, then the x faces).
+struct PmlX {  // x-face profiles of a lane's faces x..x+4
+  float a[5], k[5], zf[5];
+};
+
+__device__ __forceinline__ PmlX pml_x(const Geo<float>& g, int x, bool act) {
+  PmlX r;
+  const int D2 = (int)g.D[2];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const bool ok = act && x + j <= D2;
+    r.a[j] = ok ? g.alpha[2][x + j] : 0.f;
+    r.k[j] = ok ? g.kappa[2][x + j] : 0.f;
+    r.zf[j] = ok ? g.zf[2][x + j] : 0.f;
+  }
+  return r;
+}
+
+template <int R, int TY>
+__global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> io, int shots,
+                                                   int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lp[S::rows * kLW];
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int F2 = D2 + 1;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;
+  const bool act = x < D2;
+  const bool last_col = x + 4 == D2;
+  const float4 hx = act ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<NW_PML> dcol = damp4_col<NW_PML>(hx);
+  const float4 zcx = act ? ld4(g.zc[2] + x) : f4(0.f);
+  const PmlX px = pml_x(g, x, act);
+  const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
+  const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
+  float4 mm[S::my_rows];
+#pragma unroll
+  for (int k = 0; k < S::my_rows; ++k) {
+    const int y = y0 + r0 + k;
+    mm[k] = (act && y < D1) ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+  }
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
+    const int64_t base = (int64_t)s * g.cells;
+    const int64_t fy = (int64_t)s * g.fcells[1], fx = (int64_t)s * g.fcells[2];
+    float4 v[S::per_thread];
+    tile_load<TY, R>(io.p + base, y0, x0, D1, D2, P2, v);
+    float4 pp[S::my_rows];
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      pp[k] = (act && y < D1) ? ld4(io.p_prev + base + (y * P2 + x)) : f4(0.f);
+    }
+    __syncthreads();  // the previous shot's reads of lp are done
+    tile_store<TY, R>(lp, v);
+    __syncthreads();
+    float4 w[2 * R + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) w[k] = lrow(lp, r0 + k, lane);
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      w[2 * R] = lrow(lp, r0 + k + 2 * R, lane);
+      if (act && y < D1) {
+        const float4 nb = lnb(lp, r0 + k + R, lane);
+        float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
+        const float4 pc = w[R], pm = w[R - 1], pq = w[R + 1];
+        // y faces of rows y (own) and y + 1
+        const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
+        const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
+        const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+        const float4 oy0 = ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
+        const float4 oy1 = ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
+        float4 ny0, ny1;
+#define NW_PML_Y(c)                                                                          \
+  {                                                                                          \
+    ny0.c = __builtin_fmaf((zcx.c - zy0) * ky0, pc.c - pm.c, ay0 * oy0.c);                   \
+    ny1.c = __builtin_fmaf((zcx.c - zy1) * ky1, pq.c - pc.c, ay1 * oy1.c);                   \
+    lap.c = lap.c + ((ny1.c + oy1.c) - (ny0.c + oy0.c)) * ih1;                               \
+  }
+        NW_PML_Y(x) NW_PML_Y(y) NW_PML_Y(z) NW_PML_Y(w)
+#undef NW_PML_Y
+        st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
+        if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
+        // x faces x..x+4 of row y
+        const float zcy = g.zc[1][y];
+        const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
+        float ox[5], nx[5];
+        const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          ox[j] = (j < 4 || x + 4 <= D2) ? ox_row[j] : 0.f;
+          nx[j] = __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
+        }
+        lap.x = lap.x + ((nx[1] + ox[1]) - (nx[0] + ox[0])) * ih2;
+        lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
+        lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
+        lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
+        float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
+        if (last_col) nx_row[4] = nx[4];
+        if (io.ck_psi[1]) {
+          st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
+          if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
+          float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) cx[j] = ox[j];
+          if (last_col) cx[4] = ox[4];
+        }
+        const float hy = g.damp[1][y];
+        const Damp4<NW_PML> dm = damp4_row<NW_PML>(hy, hx, dcol);
+        const int ro = y * P2 + x;
+        const int tpv = io.taper_prev;
+        float4 o = make_float4(
+            rhs1<NW_PML>(pc.x, pp[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
+            rhs1<NW_PML>(pc.y, pp[k].y, mm[k].y, lap.y, dm.b.y, dm.tap.y, tpv),
+            rhs1<NW_PML>(pc.z, pp[k].z, mm[k].z, lap.z, dm.b.z, dm.tap.z, tpv),
+            rhs1<NW_PML>(pc.w, pp[k].w, mm[k].w, lap.w, dm.b.w, dm.tap.w, tpv));
+        const float zzr = g.dt2 * zcy;
+        o.x = o.x - (zzr * zcx.x) * pc.x;
+        o.y = o.y - (zzr * zcx.y) * pc.y;
+        o.z = o.z - (zzr * zcx.z) * pc.z;
+        o.w = o.w - (zzr * zcx.w) * pc.w;
+        for (int t = 0; t < io.src_taps; ++t) {
+          const int d = io.src_idx[s * io.src_taps + t] - ro;
+          if (d >= 0 && d < 4) {
+            const float sv = io.src_val[s * io.src_taps + t] * w_n;
+            o.x += d == 0 ? sv : 0.f;
+            o.y += d == 1 ? sv : 0.f;
+            o.z += d == 2 ? sv : 0.f;
+            o.w += d == 3 ? sv : 0.f;
+          }
+        }
+        o = mul4(dm.a, o);
+        st4(io.p_next + base + ro, o);
+        if (io.ck_prev) {
+          st4(io.ck_prev + base + ro, pp[k]);
+          st4(io.ck_cur + base + ro, pc);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) w[j] = w[j + 1];
+    }
+  }
+}
+
+template <int R, int TY>
+__global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float> io,
+                                                      int shots, int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
+  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int F2 = D2 + 1;
+  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;
+  const bool act = x < D2;
+  const bool last_col = x + 4 == D2;
+  const float4 hx = act ? ld4(g.damp[2] + x) : f4(0.f);
+  const Damp4<NW_PML> dcol = damp4_col<NW_PML>(hx);
+  const float4 zcx = act ? ld4(g.zc[2] + x) : f4(0.f);
+  const PmlX px = pml_x(g, x, act);
+  const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
+  const bool imaging = io.p_n != nullptr && io.grad != nullptr;
+  double2 ga[S::my_rows], gb[S::my_rows];
+#pragma unroll
+  for (int k = 0; k < S::my_rows; ++k) {
+    ga[k] = make_double2(0.0, 0.0);
+    gb[k] = ga[k];
+  }
+  const int s_end = min(shots, (tl.grp + 1) * spw);
+  for (int s = tl.grp * spw; s < s_end; ++s) {
+    const int64_t base = (int64_t)s * g.cells;
+    const int64_t fy = (int64_t)s * g.fcells[1], fx = (int64_t)s * g.fcells[2];
+    const float* L1 = io.lam1 + base;
+    {
+      float4 vl[S::per_thread], vm[S::per_thread];
+      tile_load<TY, R>(L1, y0, x0, D1, D2, P2, vl);
+      tile_load<TY, R>(io.ma, y0, x0, D1, D2, P2, vm);
+      __syncthreads();  // the previous shot's reads of lq / lp are done
+#pragma unroll
+      for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
+      tile_store<TY, R>(lq, vl);
+    }
+    if (imaging) {
+      float4 vp[S::per_thread];
+      tile_load<TY, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
+      tile_store<TY, R>(lp, vp);
+    }
+    __syncthreads();
+    float4 qw[2 * R + 1], pw[2 * R + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) {
+      qw[k] = lrow(lq, r0 + k, lane);
+      pw[k] = imaging ? lrow(lp, r0 + k, lane) : f4(0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
+      pw[2 * R] = imaging ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
+      if (act && y < D1) {
+        const int ro = y * P2 + x;
+        const float4 qn = lnb(lq, r0 + k + R, lane);
+        const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
+        const float hy = g.damp[1][y];
+        const Damp4<NW_PML> dm = damp4_row<NW_PML>(hy, hx, dcol);
+        const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
+        const float zcy = g.zc[1][y];
+        const float zzr = g.dt2 * zcy;
+        float4 rb, lam;
+#define NW_PML_ADJ(c)                                                                        \
+  {                                                                                          \
+    rb.c = dm.a.c * lc.c;                                                                    \
+    const float pbar = -dm.b.c * (dm.a.c * l2c.c);                                           \
+    lam.c = __builtin_fmaf(2.f - zzr * zcx.c, rb.c, pbar) + lapq.c;                          \
+  }
+        NW_PML_ADJ(x) NW_PML_ADJ(y) NW_PML_ADJ(z) NW_PML_ADJ(w)
+#undef NW_PML_ADJ
+        float4 lapp = f4(0.f);
+        if (imaging) {
+          const float4 pn = lnb(lp, r0 + k + R, lane);
+          lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
+        }
+        // y faces of rows y and y + 1
+        {
+          const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
+          const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
+          const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
+          const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+          const float4 i0 = ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
+          const float4 i1 = ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
+          float4 o0, o1;
+#define NW_PML_ADJ_Y(c)                                                                      \
+  {                                                                                          \
+    const float dq0 = qc.c - qm.c, dq1 = qp.c - qc.c;                                        \
+    const float t0 = i0.c - ih1 * dq0, t1 = i1.c - ih1 * dq1;                                \
+    const float b0 = (zcx.c - zy0) * ky0, b1 = (zcx.c - zy1) * ky1;                          \
+    lam.c = lam.c - (b1 * t1 - b0 * t0);                                                     \
+    o0.c = ay0 * t0 - ih1 * dq0;                                                             \
+    o1.c = ay1 * t1 - ih1 * dq1;                                                             \
+  }
+          NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
+#undef NW_PML_ADJ_Y
+          st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
+          if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
+          if (imaging) {
+            const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
+            const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
+            const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
+            const float4 w1 = ld4(io.psi_low[1] + fy + ((int64_t)(y + 1) * D2 + x));
+            lapp.x = lapp.x + ((h1.x + w1.x) - (h0.x + w0.x)) * ih1;
+            lapp.y = lapp.y + ((h1.y + w1.y) - (h0.y + w0.y)) * ih1;
+            lapp.z = lapp.z + ((h1.z + w1.z) - (h0.z + w0.z)) * ih1;
+            lapp.w = lapp.w + ((h1.w + w1.w) - (h0.w + w0.w)) * ih1;
+          }
+        }
+        // x faces x..x+4 of row y
+        {
+          const float4 qc = qw[R];
+          const float qf[6] = {qn.y, qc.x, qc.y, qc.z, qc.w, qn.z};  // q at x-1 .. x+4
+          const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
+          float t[5], dq[5], b[5];
+#pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            const float in = (j < 4 || x + 4 <= D2) ? in_row[j] : 0.f;
+            dq[j] = qf[j + 1] - qf[j];
+            t[j] = in - ih2 * dq[j];
+            b[j] = (zcy - px.zf[j]) * px.k[j];
+          }
+          lam.x = lam.x - (b[1] * t[1] - b[0] * t[0]);
+          lam.y = lam.y - (b[2] * t[2] - b[1] * t[1]);
+          lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
+          lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
+          float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
+          if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
+          if (imaging) {
+            const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
+            const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
+            float hw[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) hw[j] = hr[j] + wr[j];
+            lapp.x = lapp.x + (hw[1] - hw[0]) * ih2;
+            lapp.y = lapp.y + (hw[2] - hw[1]) * ih2;
+            lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
+            lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
+          }
+        }
+        st4(io.lam0 + base + ro, lam);
+        if (imaging) {
+          ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
+          ga[k].y = __builtin_fma((double)rb.y, (double)lapp.y, ga[k].y);
+          gb[k].x = __builtin_fma((double)rb.z, (double)lapp.z, gb[k].x);
+          gb[k].y = __builtin_fma((double)rb.w, (double)lapp.w, gb[k].y);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) {
+        qw[j] = qw[j + 1];
+        pw[j] = pw[j + 1];
+      }
+    }
+  }
+  if (imaging) {
+    double* G = io.grad + (int64_t)tl.grp * g.cells;
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      if (act && y < D1) {
+        double2* gp = reinterpret_cast<double2*>(G + (y * P2 + x));
+        double2 a = gp[0], b = gp[1];
+        a.x += ga[k].x;
+        a.y += ga[k].y;
+        b.x += gb[k].x;
+        b.y += gb[k].y;
+        gp[0] = a;
+        gp[1] = b;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype) {
-  return g.nd == 2 && dtype == NW_F32 && absorbing != NW_PML && g.D[2] % 4 == 0;
+  (void)absorbing;
+  return g.nd == 2 && dtype == NW_F32 && g.D[2] % 4 == 0;
 }
 
 // 1-D grid of kXcds * ceil(tiles / kXcds) workgroups (tile_of maps them, a few return)
@@ -1269,32 +1618,53 @@ static unsigned grid_tiles3(const Geo<float>& g, int ty, int shots, int spw) {
   return (unsigned)(kXcds * div_up(T, kXcds));
 }
 
+template <int TY>
+constexpr int lds_rows() {
+  return TY < 8 ? 8 : (TY > 32 ? 32 : TY);
+}
+
+// PML plans always step with the LDS-tiled face kernels, 8 or 16 rows (the 32-row variants
+// need more than 256 VGPRs)
+template <int TY>
+constexpr int pml_rows() {
+  return TY <= 8 ? 8 : 16;
+}
+
 template <int R, int ABS, int TY>
 static void launch_step_t(const Geo<float>& g, const StepIO<float>& io, int shots, int spw,
                           int xm, hipStream_t st) {
   // xm bit 2 selects the LDS-tiled kernel, bit 1 the two-rows-per-iteration march
-  if ((xm & 4) && TY >= 8 && TY <= 32)
-    hipLaunchKernelGGL((k3_step<R, ABS, (TY < 8 ? 8 : (TY > 32 ? 32 : TY))>),
-                       dim3(grid_tiles3(g, TY, shots, spw)), dim3(256), 0, st, g, io, shots, spw,
-                       xm & 1);
-  else if (xm & 2)
+  if constexpr (ABS == NW_PML) {
+    hipLaunchKernelGGL((k3_step_pml<R, pml_rows<TY>()>),
+                       dim3(grid_tiles3(g, pml_rows<TY>(), shots, spw)), dim3(256), 0, st, g, io,
+                       shots, spw, xm & 1);
+  } else if ((xm & 4) && TY >= 8 && TY <= 32) {
+    hipLaunchKernelGGL((k3_step<R, ABS, lds_rows<TY>()>), dim3(grid_tiles3(g, TY, shots, spw)),
+                       dim3(256), 0, st, g, io, shots, spw, xm & 1);
+  } else if (xm & 2) {
     hipLaunchKernelGGL((k2_step<R, ABS, TY, 2>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);
-  else
+  } else {
     hipLaunchKernelGGL((k2_step<R, ABS, TY, 1>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);
+  }
 }
 
 template <int R, int ABS, int TY>
 static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int spw, int xm,
                          hipStream_t st) {
-  if ((xm & 4) && TY >= 8 && TY <= 32)
-    hipLaunchKernelGGL((k3_adjoint<R, ABS, (TY < 8 ? 8 : (TY > 32 ? 32 : TY))>),
+  if constexpr (ABS == NW_PML) {
+    hipLaunchKernelGGL((k3_adjoint_pml<R, pml_rows<TY>()>),
+                       dim3(grid_tiles3(g, pml_rows<TY>(), shots, spw)), dim3(256), 0, st, g, io,
+                       shots, spw, xm & 1);
+  } else if ((xm & 4) && TY >= 8 && TY <= 32) {
+    hipLaunchKernelGGL((k3_adjoint<R, ABS, lds_rows<TY>()>),
                        dim3(grid_tiles3(g, TY, shots, spw)), dim3(256), 0, st, g, io, shots, spw,
                        xm & 1);
-  else
+  } else {
     hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);
+  }
 }
 
 template <int R, int ABS>
@@ -1331,7 +1701,9 @@ void fast2d_step(const Geo<float>& g, int order, int absorbing, const StepIO<flo
     case NW_SPONGE * 2 + 0: step_ty<1, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
     case NW_SPONGE * 2 + 1: step_ty<2, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
     case NW_CERJAN * 2 + 0: step_ty<1, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
-    default: step_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    case NW_CERJAN * 2 + 1: step_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    case NW_PML * 2 + 0: step_ty<1, NW_PML>(g, io, shots, ty, spw, xm, st); break;
+    default: step_ty<2, NW_PML>(g, io, shots, ty, spw, xm, st); break;
   }
 }
 
@@ -1345,7 +1717,9 @@ void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<f
     case NW_SPONGE * 2 + 0: adj_ty<1, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
     case NW_SPONGE * 2 + 1: adj_ty<2, NW_SPONGE>(g, io, shots, ty, spw, xm, st); break;
     case NW_CERJAN * 2 + 0: adj_ty<1, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
-    default: adj_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    case NW_CERJAN * 2 + 1: adj_ty<2, NW_CERJAN>(g, io, shots, ty, spw, xm, st); break;
+    case NW_PML * 2 + 0: adj_ty<1, NW_PML>(g, io, shots, ty, spw, xm, st); break;
+    default: adj_ty<2, NW_PML>(g, io, shots, ty, spw, xm, st); break;
   }
 }
 
@@ -1397,7 +1771,8 @@ void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& 
     case NW_SPONGE * 2 + 0: steps_ty<1, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
     case NW_SPONGE * 2 + 1: steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
     case NW_CERJAN * 2 + 0: steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
-    default: steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_CERJAN * 2 + 1: steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    default: break;  // PML plans are never blocked (their face fields are not carried)
   }
 }
 
@@ -1441,7 +1816,8 @@ void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const A
     case NW_SPONGE * 2 + 0: adj_steps_ty<1, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
     case NW_SPONGE * 2 + 1: adj_steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
     case NW_CERJAN * 2 + 0: adj_steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
-    default: adj_steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    case NW_CERJAN * 2 + 1: adj_steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
+    default: break;  // PML plans are never blocked
   }
 }
 

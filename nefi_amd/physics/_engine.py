@@ -192,7 +192,7 @@ class DevicePlan:
             n = S
             for b in range(3):
                 n *= host.dims[b] + (b == a)
-            self.face_numel.append(n)
+            self.face_numel.append(-(-n // 64) * 64)  # the library's 64-element rounding
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order varies
         h = getattr(self, "handle", None)

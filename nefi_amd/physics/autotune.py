@@ -28,6 +28,7 @@ from ..errors import NativeError
 
 ROWS = (4, 8, 16, 32, 64)
 LDS_ROWS = (8, 16, 32)
+PML_ROWS = (8, 16)  # PML plans step with LDS-tiled face kernels of these heights only
 GROUPS = (1, 2, 4, 8, 16)
 DEPTHS = (2, 3, 4)  # temporal blocking depths
 BLOCK_STEPS = 12  # forward steps per timing: a multiple of every depth
@@ -73,9 +74,11 @@ def plan_key(plan) -> str:
     return f"v3|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
 
 
-def candidates(kind: int, n_shots: int):
+def candidates(kind: int, n_shots: int, pml: bool = False):
     """(rows, shots per group, rows loaded per iteration, LDS tiles, blocking depth) to try."""
     groups = [g for g in GROUPS if g <= n_shots]
+    if pml:  # no register-march or blocked variants carry the face fields
+        return [(r, g, 1, 1, 1) for r in PML_ROWS for g in groups]
     out = [(r, g, u, 0, 1) for r in ROWS for g in groups
            for u in ((1, 2) if kind == 0 else (1,))]
     out += [(r, g, 1, 1, 1) for r in LDS_ROWS for g in groups]
@@ -132,9 +135,10 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
     h = plan.host
     if h.cells * h.n_shots < MIN_WORK and not force:
         # launch-bound: no timing, but fewer launches (4 steps per forward launch)
-        _apply(plan, 0, *SMALL_STEP)
+        small = (16, 1, 1, 1, 1) if plan.pml else SMALL_STEP
+        _apply(plan, 0, *small)
         return {"step": dict(zip(("rows", "shots_per_group", "rows_per_iter", "lds", "depth"),
-                                 SMALL_STEP)), "static": True}
+                                 small)), "static": True}
     key = plan_key(plan)
     table = _load()
     if key in table and not force:
@@ -148,7 +152,7 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
             for kind, name in ((0, "step"), (1, "adjoint")):
                 timings = {}
                 n = min(BLOCK_STEPS, max(1, h.n_steps))
-                for cand in candidates(kind, h.n_shots):
+                for cand in candidates(kind, h.n_shots, plan.pml):
                     try:
                         _time_kind(plan, kind, cand, 1)  # warm the candidate once
                         timings[cand] = _time_kind(plan, kind, cand, n)

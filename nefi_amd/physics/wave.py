@@ -539,7 +539,8 @@ class _LeapfrogBase(Operator):
                 for a, axis in enumerate(h.used_axes):
                     fs = list(self.grid.padded_shape)
                     fs[axis - (3 - self.grid.d)] += 1
-                    out.append(psi[a].view(h.n_shots, *fs))
+                    n_face = h.n_shots * math.prod(fs)  # buffers are rounded up past this
+                    out.append(psi[a][:n_face].view(h.n_shots, *fs))
             return tuple(t.to(device=out_device, dtype=out_dtype) for t in out)
 
         obs = []

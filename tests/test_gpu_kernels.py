@@ -62,7 +62,7 @@ def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1, keep_lap=1, depth_ad
 
 
 @pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2),
-                                             ("sponge", 2)])
+                                             ("sponge", 2), ("pml", 4), ("pml", 2)])
 def test_fast_kernels_match_generic_over_tiles(absorbing, order):
     dom, geo, c, kw = _survey(absorbing, order)
     print(f"[case fast-vs-generic {absorbing} o{order}]", flush=True)
@@ -228,8 +228,10 @@ def test_blocked_reverse_initial_condition_2d():
         assert np.array_equal(grads[depth_adj], grads[1])
 
 
-def test_fast_path_against_oracle_uneven_shots():
-    dom, geo, c, kw = _survey("sponge", 4, n=(40, 88), shots=3, n_t=50)
+@pytest.mark.parametrize("absorbing", ["sponge", "pml"])
+def test_fast_path_against_oracle_uneven_shots(absorbing):
+    print(f"[case oracle uneven shots {absorbing}]", flush=True)
+    dom, geo, c, kw = _survey(absorbing, 4, n=(40, 88), shots=3, n_t=50)
     op = ShotBatchWaveOperator(dom, geo, **kw)
     rows = sum(len(r) for _, r in geo)
     w = np.random.default_rng(4).standard_normal((rows, kw["n_t"]))
@@ -246,16 +248,20 @@ def test_fast_path_against_oracle_uneven_shots():
         r0 += len(rec)
     yo = np.concatenate(ys, 0)
     ey, eg = rel(y, yo), rel(g, gs)
-    print(f"uneven shots vs FP64 oracle: traces {ey:.2e}, gradient {eg:.2e}", flush=True)
+    print(f"{absorbing} uneven shots vs FP64 oracle: traces {ey:.2e}, gradient {eg:.2e}",
+          flush=True)
     assert ey < 1e-5 and eg < 1e-4
 
 
-def test_hip_graph_replays_match_eager():
+@pytest.mark.parametrize("absorbing", ["sponge", "pml"])
+def test_hip_graph_replays_match_eager(absorbing):
     """Every call captured once, then replayed: each replay (with eager work and syncs between
-    them) must reproduce the eager traces and gradient exactly."""
+    them) must reproduce the eager traces and gradient exactly (PML: the face fields' buffer
+    pairs swap like the wavefields')."""
     from nefi_amd import _native as nat
 
-    dom, geo, c, kw = _survey("sponge", 4, n=(40, 88), shots=3, n_t=50)
+    print(f"[case graph replays {absorbing}]", flush=True)
+    dom, geo, c, kw = _survey(absorbing, 4, n=(40, 88), shots=3, n_t=50)
     op = ShotBatchWaveOperator(dom, geo, **kw)
     rows = sum(len(r) for _, r in geo)
     w = torch.randn(rows, kw["n_t"], device=DEV, generator=torch.Generator(DEV).manual_seed(5))
