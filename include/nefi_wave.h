@@ -272,16 +272,18 @@ int32_t nw_clamp(nw_plan* plan, void* p, const int32_t* cells, const void* value
  * waits for the recorded events and returns the summed duration (ms) and launch count of one
  * kernel kind since profiling was enabled: 0 = forward step, 1 = reverse step,
  * 2 = receiver sample, 3 = residual inject, 4 = temporally blocked forward steps,
- * 5 = temporally blocked reverse steps.
+ * 5 = temporally blocked reverse steps, 6 = trace filter (nefi_filter.h).
  * nw_profile_units returns the time steps those launches advanced (for kind 2, the time
- * samples recorded). Enabling clears earlier records. Process-wide; not thread-safe. */
+ * samples recorded; for kind 6, the traces filtered). Enabling clears earlier records.
+ * Process-wide; not thread-safe. */
 enum nw_kernel_kind {
   NW_K_STEP = 0,
   NW_K_ADJOINT = 1,
   NW_K_SAMPLE = 2,
   NW_K_INJECT = 3,
   NW_K_STEPS = 4,
-  NW_K_ADJOINT_STEPS = 5
+  NW_K_ADJOINT_STEPS = 5,
+  NW_K_FILTER = 6
 };
 int32_t nw_profile_enable(int32_t on);
 int32_t nw_profile_read(int32_t kind, double* total_ms, int64_t* launches);

@@ -1,4 +1,5 @@
-"""ctypes binding of ``libnefi_wave.so`` (declared in ``include/nefi_wave.h``).
+"""ctypes binding of ``libnefi_wave.so`` (declared in ``include/nefi_wave.h`` and
+``include/nefi_filter.h``).
 
 The library is loaded after ``torch`` so that it binds to the HIP runtime torch already loaded
 (both carry the soname ``libamdhip64.so.7``); torch's streams and device pointers are then valid
@@ -50,6 +51,13 @@ EXPORTS = (
     "nw_grad_slices",
     "nw_plan_set_option",
     "nw_plan_get_option",
+)
+# every symbol include/nefi_filter.h declares
+FILTER_EXPORTS = (
+    "nw_filter_create",
+    "nw_filter_destroy",
+    "nw_filter_length",
+    "nw_filter_apply",
 )
 
 
@@ -129,6 +137,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_grad_slices": (i64, [vp]),
         "nw_plan_set_option": (i32, [vp, i32, i32]),
         "nw_plan_get_option": (i32, [vp, i32, POINTER(c_int32)]),
+        "nw_filter_create": (i32, [i32, i64, POINTER(c_double), POINTER(c_void_p)]),
+        "nw_filter_destroy": (i32, [vp]),
+        "nw_filter_length": (i64, [vp]),
+        "nw_filter_apply": (i32, [vp, vp, vp, vp, vp, i64, i32, c_double, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -163,7 +175,7 @@ def lib() -> ctypes.CDLL:
     return handle
 
 
-K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT, K_STEPS, K_ADJOINT_STEPS = 0, 1, 2, 3, 4, 5
+K_STEP, K_ADJOINT, K_SAMPLE, K_INJECT, K_STEPS, K_ADJOINT_STEPS, K_FILTER = 0, 1, 2, 3, 4, 5, 6
 OPT_XCD_MAP = 0
 OPT_ROWS_PER_ITER = 1
 OPT_LDS_TILES_STEP = 2

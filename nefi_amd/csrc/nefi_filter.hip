@@ -1,0 +1,408 @@
+// Trace filters and the fused trace misfit: the C ABI of include/nefi_filter.h.
+//
+// The reference's trace filters (nefi/instances/wave_fwi/losses.py:29-50,
+// nefi/instances/photoacoustic3d/sensor.py:24-38) multiply the zero-padded real FFT of each
+// trace by a real, even response and crop. For outputs t < n that is the linear convolution of
+// the trace with the symmetric kernel g[k] = h[|k|], |k| < n, h = irfft(H, N)[:n]; on any
+// circular length L >= 2n - 1 the circular convolution with g (wrapped) gives the same first n
+// samples. A plan stores the spectrum G of that wrapped kernel for a power of two L, real
+// because g is even, in bit-reversed order and divided by L.
+//
+// One workgroup filters two traces at a time as the real and imaginary parts of one complex
+// sequence: an in-place decimation-in-frequency FFT in LDS leaves the spectrum in bit-reversed
+// order, the product with G (real) keeps the two traces apart, and an in-place decimation-in-time
+// inverse FFT takes the bit-reversed spectrum back to natural order, so no permutation pass is
+// needed. The residual (x - x_sub), the mask, the output scale and the float64 sums of the
+// masked squares are fused into the load and store passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nefi_filter.h"
+#include "nefi_wave.h"
+#include "nefi_wave_internal.h"
+
+struct nw_filter {
+  int dtype;
+  int64_t n;
+  int64_t L;
+  int log2L;
+  void* spec;        // L reals: G in bit-reversed order, / L
+  void* tw;          // L/2 complex twiddles exp(-2 pi i k / L)
+  double* partial;   // two float64 sums per workgroup of the last apply
+};
+
+namespace nw {
+namespace {
+
+constexpr int kFilterThreads = 256;
+constexpr int kFilterBlocks = 2048;  // workgroups of one apply (a loop over row pairs)
+constexpr int kMinLog2 = 6;
+constexpr int kMaxLog2F32 = 14;  // 16384 float2 = 128 KiB of LDS
+constexpr int kMaxLog2F64 = 13;  // 8192 double2 = 128 KiB
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+template <typename T>
+struct Cplx;
+template <>
+struct Cplx<float> {
+  using type = float2;
+};
+template <>
+struct Cplx<double> {
+  using type = double2;
+};
+
+template <typename T>
+struct FilterIO {
+  const T* x;
+  const T* xs;    // subtracted input (or null)
+  const T* mask;  // weights of x's shape (or null)
+  T* y;           // output (or null)
+  const T* spec;
+  const typename Cplx<T>::type* tw;
+  int64_t rows;
+  int64_t n;
+  T scale;
+  double* partial;
+};
+
+template <typename V>
+__device__ __forceinline__ V cmul(const V& a, const V& b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+template <typename V>
+__device__ __forceinline__ V cmul_conj(const V& a, const V& b) {  // a * conj(b)
+  V r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.y * b.x - a.x * b.y;
+  return r;
+}
+
+// the two float64 sums of a workgroup (fixed order: lanes by shuffle tree, then wavefronts)
+__device__ __forceinline__ void block_sums(double v, double m, double* out) {
+  __shared__ double red[2][kFilterThreads / 64];
+  for (int off = 32; off >= 1; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+    m += __shfl_xor(m, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = v;
+    red[1][wave] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kFilterThreads / 64; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+template <typename T, int LOG2L>
+__global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
+  using V = typename Cplx<T>::type;
+  constexpr int L = 1 << LOG2L;
+  __shared__ V z[L];
+  const int n = (int)io.n;
+  double acc_v = 0.0, acc_m = 0.0;
+  const int64_t pairs = (io.rows + 1) / 2;
+  for (int64_t p = blockIdx.x; p < pairs; p += gridDim.x) {
+    const int64_t r0 = 2 * p;
+    const bool two = r0 + 1 < io.rows;
+    const int64_t o0 = r0 * io.n, o1 = two ? o0 + io.n : o0;
+    __syncthreads();  // the previous pair's reads of z are done
+    for (int t = threadIdx.x; t < L; t += kFilterThreads) {
+      T a = T(0), b = T(0);
+      if (t < n) {
+        a = io.x[o0 + t];
+        if (io.xs) a = a - io.xs[o0 + t];
+        if (two) {
+          b = io.x[o1 + t];
+          if (io.xs) b = b - io.xs[o1 + t];
+        }
+      }
+      V v;
+      v.x = a;
+      v.y = b;
+      z[t] = v;
+    }
+    __syncthreads();
+    // forward transform, decimation in frequency: natural order in, bit-reversed out
+#pragma unroll 1
+    for (int lh = LOG2L - 1; lh >= 0; --lh) {
+      const int h = 1 << lh, ts = L >> (lh + 1);
+      for (int b = threadIdx.x; b < L / 2; b += kFilterThreads) {
+        const int j = b & (h - 1);
+        const int i = 2 * b - j;
+        const V u = z[i], v = z[i + h];
+        V s, d;
+        s.x = u.x + v.x;
+        s.y = u.y + v.y;
+        d.x = u.x - v.x;
+        d.y = u.y - v.y;
+        z[i] = s;
+        z[i + h] = cmul(d, io.tw[j * ts]);
+      }
+      __syncthreads();
+    }
+    for (int t = threadIdx.x; t < L; t += kFilterThreads) {
+      const T g = io.spec[t];
+      V v = z[t];
+      v.x = v.x * g;
+      v.y = v.y * g;
+      z[t] = v;
+    }
+    __syncthreads();
+    // inverse transform, decimation in time: bit-reversed in, natural order out
+#pragma unroll 1
+    for (int lh = 0; lh < LOG2L; ++lh) {
+      const int h = 1 << lh, ts = L >> (lh + 1);
+      for (int b = threadIdx.x; b < L / 2; b += kFilterThreads) {
+        const int j = b & (h - 1);
+        const int i = 2 * b - j;
+        const V u = z[i];
+        const V v = cmul_conj(z[i + h], io.tw[j * ts]);
+        V s, d;
+        s.x = u.x + v.x;
+        s.y = u.y + v.y;
+        d.x = u.x - v.x;
+        d.y = u.y - v.y;
+        z[i] = s;
+        z[i + h] = d;
+      }
+      __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n; t += kFilterThreads) {
+      const V v = z[t];
+      const T m0 = io.mask ? io.mask[o0 + t] : T(1);
+      if (io.y) io.y[o0 + t] = io.scale * (m0 * v.x);
+      acc_v += (double)m0 * ((double)v.x * (double)v.x);
+      acc_m += (double)m0;
+      if (two) {
+        const T m1 = io.mask ? io.mask[o1 + t] : T(1);
+        if (io.y) io.y[o1 + t] = io.scale * (m1 * v.y);
+        acc_v += (double)m1 * ((double)v.y * (double)v.y);
+        acc_m += (double)m1;
+      }
+    }
+  }
+  if (io.partial) block_sums(acc_v, acc_m, io.partial + 2 * blockIdx.x);
+}
+
+// the same passes without the transform (filter = 0)
+template <typename T>
+__global__ void __launch_bounds__(kFilterThreads) k_filter_ident(FilterIO<T> io) {
+  const int64_t total = io.rows * io.n;
+  double acc_v = 0.0, acc_m = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kFilterThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kFilterThreads) {
+    T v = io.x[e];
+    if (io.xs) v = v - io.xs[e];
+    const T m = io.mask ? io.mask[e] : T(1);
+    if (io.y) io.y[e] = io.scale * (m * v);
+    acc_v += (double)m * ((double)v * (double)v);
+    acc_m += (double)m;
+  }
+  if (io.partial) block_sums(acc_v, acc_m, io.partial + 2 * blockIdx.x);
+}
+
+// sums[k] = sum over workgroups of partial[2 w + k], in a fixed tree order
+__global__ void __launch_bounds__(kFilterThreads) k_filter_sums(const double* partial, int blocks,
+                                                               double* sums) {
+  double v = 0.0, m = 0.0;
+  for (int w = threadIdx.x; w < blocks; w += kFilterThreads) {
+    v += partial[2 * w];
+    m += partial[2 * w + 1];
+  }
+  block_sums(v, m, sums);
+}
+
+// host: in-place radix-2 complex FFT (forward, e^{-i}), double
+void host_fft(std::vector<std::complex<double>>& a) {
+  const size_t L = a.size();
+  for (size_t i = 1, j = 0; i < L; ++i) {
+    size_t bit = L >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= L; len <<= 1) {
+    for (size_t k = 0; k < len / 2; ++k) {
+      const double ang = -2.0 * kPi * (double)k / (double)len;
+      const std::complex<double> w(std::cos(ang), std::sin(ang));
+      for (size_t i = 0; i < L; i += len) {
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+size_t bitrev(size_t i, int bits) {
+  size_t r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+template <typename T>
+int upload(nw_filter* f, const std::vector<double>& spec, const std::vector<double>& tw) {
+  std::vector<T> s(spec.begin(), spec.end()), t(tw.begin(), tw.end());
+  if (hipMalloc(&f->spec, s.size() * sizeof(T)) != hipSuccess ||
+      hipMalloc(&f->tw, t.size() * sizeof(T)) != hipSuccess ||
+      hipMalloc((void**)&f->partial, 2 * kFilterBlocks * sizeof(double)) != hipSuccess)
+    return fail(NW_ERR_HIP, "nw_filter_create: device allocation failed");
+  if (hipMemcpy(f->spec, s.data(), s.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(f->tw, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(NW_ERR_HIP, "nw_filter_create: copy to device failed");
+  return NW_OK;
+}
+
+template <typename T, int LOG2L>
+void launch_fft(const FilterIO<T>& io, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_filter_fft<T, LOG2L>), dim3(blocks), dim3(kFilterThreads), 0, st, io);
+}
+
+template <typename T>
+void dispatch_fft(int log2L, const FilterIO<T>& io, int blocks, hipStream_t st) {
+  switch (log2L) {
+    case 6: launch_fft<T, 6>(io, blocks, st); break;
+    case 7: launch_fft<T, 7>(io, blocks, st); break;
+    case 8: launch_fft<T, 8>(io, blocks, st); break;
+    case 9: launch_fft<T, 9>(io, blocks, st); break;
+    case 10: launch_fft<T, 10>(io, blocks, st); break;
+    case 11: launch_fft<T, 11>(io, blocks, st); break;
+    case 12: launch_fft<T, 12>(io, blocks, st); break;
+    case 13: launch_fft<T, 13>(io, blocks, st); break;
+    default:
+      if constexpr (sizeof(T) == 4) launch_fft<T, 14>(io, blocks, st);
+      break;
+  }
+}
+
+template <typename T>
+int apply_t(nw_filter* f, const void* x, const void* xs, const void* mask, void* y, int64_t rows,
+            int filter, double scale, double* sums, hipStream_t st) {
+  FilterIO<T> io;
+  io.x = (const T*)x;
+  io.xs = (const T*)xs;
+  io.mask = (const T*)mask;
+  io.y = (T*)y;
+  io.spec = (const T*)f->spec;
+  io.tw = (const typename Cplx<T>::type*)f->tw;
+  io.rows = rows;
+  io.n = f->n;
+  io.scale = (T)scale;
+  io.partial = sums ? f->partial : nullptr;
+  int blocks = 0;
+  if (rows > 0) {
+    const int pid = prof_begin(NW_K_FILTER, st, rows);
+    if (filter) {
+      blocks = (int)std::min<int64_t>((rows + 1) / 2, kFilterBlocks);
+      dispatch_fft<T>(f->log2L, io, blocks, st);
+    } else {
+      const int64_t chunks = (rows * f->n + 4 * kFilterThreads - 1) / (4 * kFilterThreads);
+      blocks = (int)std::min<int64_t>(chunks, kFilterBlocks);
+      hipLaunchKernelGGL((k_filter_ident<T>), dim3(blocks), dim3(kFilterThreads), 0, st, io);
+    }
+    prof_end(pid, st);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(NW_ERR_HIP, std::string("k_filter: ") + hipGetErrorString(e));
+  }
+  if (sums) {  // (no partials when rows == 0: the sums are zero)
+    hipLaunchKernelGGL(k_filter_sums, dim3(1), dim3(kFilterThreads), 0, st, f->partial, blocks,
+                       sums);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return fail(NW_ERR_HIP, std::string("k_filter_sums: ") + hipGetErrorString(e));
+  }
+  return NW_OK;
+}
+
+}  // namespace
+}  // namespace nw
+
+using nw::fail;
+
+extern "C" {
+
+int32_t nw_filter_create(int32_t dtype, int64_t n, const double* taps, nw_filter** out) {
+  if (!out || !taps) return fail(NW_ERR_ARG, "nw_filter_create: null argument");
+  *out = nullptr;
+  if (dtype != NW_F32 && dtype != NW_F64) return fail(NW_ERR_ARG, "nw_filter_create: bad dtype");
+  const int64_t max_n = dtype == NW_F32 ? NW_FILTER_MAX_N_F32 : NW_FILTER_MAX_N_F64;
+  if (n < 1 || n > max_n)
+    return fail(NW_ERR_UNSUPPORTED, "nw_filter_create: trace length must be in [1, " +
+                                        std::to_string(max_n) + "]");
+  int log2L = nw::kMinLog2;
+  while (((int64_t)1 << log2L) < 2 * n - 1) ++log2L;
+  const int64_t L = (int64_t)1 << log2L;
+  // the wrapped even kernel and its spectrum (real up to rounding)
+  std::vector<std::complex<double>> g((size_t)L, 0.0);
+  g[0] = taps[0];
+  for (int64_t k = 1; k < n; ++k) {
+    g[(size_t)k] = taps[k];
+    g[(size_t)(L - k)] = taps[k];
+  }
+  nw::host_fft(g);
+  std::vector<double> spec((size_t)L), tw((size_t)L);
+  for (int64_t p = 0; p < L; ++p) spec[(size_t)p] = g[nw::bitrev((size_t)p, log2L)].real() / (double)L;
+  for (int64_t k = 0; k < L / 2; ++k) {
+    const double ang = -2.0 * nw::kPi * (double)k / (double)L;
+    tw[(size_t)(2 * k)] = std::cos(ang);
+    tw[(size_t)(2 * k + 1)] = std::sin(ang);
+  }
+  nw_filter* f = new nw_filter();
+  f->dtype = dtype;
+  f->n = n;
+  f->L = L;
+  f->log2L = log2L;
+  f->spec = nullptr;
+  f->tw = nullptr;
+  f->partial = nullptr;
+  const int rc = dtype == NW_F32 ? nw::upload<float>(f, spec, tw) : nw::upload<double>(f, spec, tw);
+  if (rc != NW_OK) {
+    nw_filter_destroy(f);
+    return rc;
+  }
+  *out = f;
+  return NW_OK;
+}
+
+int32_t nw_filter_destroy(nw_filter* f) {
+  if (!f) return NW_OK;
+  for (void* p : {f->spec, f->tw, (void*)f->partial})
+    if (p) (void)hipFree(p);
+  delete f;
+  return NW_OK;
+}
+
+int64_t nw_filter_length(const nw_filter* f) { return f ? f->L : 0; }
+
+int32_t nw_filter_apply(nw_filter* f, const void* x, const void* x_sub, const void* mask,
+                        void* y, int64_t rows, int32_t filter, double scale, double* sums,
+                        void* stream) {
+  if (!f || (!x && rows > 0)) return fail(NW_ERR_ARG, "nw_filter_apply: null argument");
+  if (rows < 0) return fail(NW_ERR_ARG, "nw_filter_apply: rows must be >= 0");
+  if (rows * f->n >= ((int64_t)1 << 40)) return fail(NW_ERR_ARG, "nw_filter_apply: too many samples");
+  hipStream_t st = (hipStream_t)stream;
+  if (f->dtype == NW_F32)
+    return nw::apply_t<float>(f, x, x_sub, mask, y, rows, filter, scale, sums, st);
+  return nw::apply_t<double>(f, x, x_sub, mask, y, rows, filter, scale, sums, st);
+}
+
+}  // extern "C"

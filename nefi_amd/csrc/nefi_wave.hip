@@ -78,7 +78,7 @@ struct GraphEntry {
 };
 
 // brackets one launch; returns the record index or -1 when timing is off
-static int prof_begin(int kind, hipStream_t st, int64_t units = 1) {
+int prof_begin(int kind, hipStream_t st, int64_t units) {
   if (!g_prof.on) return -1;
   ProfRec r{kind, units, g_prof.get(), g_prof.get()};
   if (!r.start || !r.stop) return -1;
@@ -87,7 +87,7 @@ static int prof_begin(int kind, hipStream_t st, int64_t units = 1) {
   return (int)g_prof.recs.size() - 1;
 }
 
-static void prof_end(int id, hipStream_t st) {
+void prof_end(int id, hipStream_t st) {
   if (id >= 0) hipEventRecord(g_prof.recs[id].stop, st);
 }
 

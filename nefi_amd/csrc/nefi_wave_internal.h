@@ -1,6 +1,8 @@
 // Kernel parameter blocks shared by the generic and the 2-D kernels of libnefi_wave.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 
@@ -8,6 +10,9 @@ namespace nw {
 
 extern thread_local std::string g_err;
 int fail(int code, const std::string& msg);
+// per-launch timing brackets (nw_profile_enable): record index, or -1 when timing is off
+int prof_begin(int kind, hipStream_t st, int64_t units = 1);
+void prof_end(int id, hipStream_t st);
 
 // Geometry and constants of a plan, passed by value to every kernel.
 template <typename T>

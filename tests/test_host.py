@@ -35,9 +35,15 @@ ROOT = Path(__file__).resolve().parent.parent
 
 # ---- C ABI ---------------------------------------------------------------------------------
 def test_library_loads_and_exports_every_declared_symbol():
-    header = (ROOT / "include" / "nefi_wave.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int32_t|int64_t|const char\*)\s+(nw_\w+)\(", header, re.M))
-    assert declared == set(nat.EXPORTS)
+    def declared_in(name):
+        text = (ROOT / "include" / name).read_text()
+        return set(re.findall(r"^\s*(?:int32_t|int64_t|const char\*)\s+(nw_\w+)\(", text, re.M))
+
+    assert declared_in("nefi_wave.h") == set(nat.EXPORTS)
+    assert declared_in("nefi_filter.h") == set(nat.FILTER_EXPORTS)
+    assert sorted(p.name for p in (ROOT / "include").glob("*.h")) == ["nefi_filter.h",
+                                                                     "nefi_wave.h"]
+    declared = set(nat.EXPORTS) | set(nat.FILTER_EXPORTS)
     lib = nat.lib()
     for name in declared:
         assert getattr(lib, name) is not None
