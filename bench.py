@@ -39,9 +39,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"step": 16.0, "steps_blocked": 16.0, "adjoint": 28.0,
                     "adjoint_blocked": 28.0}
+# PML plans (--absorbing pml): the step also reads and writes both face fields (28 B); the
+# reverse step reads the adjoint face fields and both psi levels of the imaging term (52 B)
+BYTES_PER_UPDATE_PML = {"step": 28.0, "adjoint": 52.0}
 
 
-def survey(n=1024, shots=64, n_t=4001, seed=0):
+def survey(n=1024, shots=64, n_t=4001, seed=0, absorbing="sponge"):
     """Geometry and model of the uneven 64-shot survey (host, float64)."""
     from scipy.ndimage import gaussian_filter
 
@@ -61,7 +64,7 @@ def survey(n=1024, shots=64, n_t=4001, seed=0):
         rec = [[(i + 0.5) * L / r, 0.5 * dx] for i in range(r)]
         shot_geo.append((src, rec))
     kw = dict(n_t=n_t, dt_obs=0.4 * dx / c_max, f0=15.0, c_max=c_max, order=4,
-              absorbing="sponge", absorb_width=20 * dx)
+              absorbing=absorbing, absorb_width=20 * dx)
     return dict(n=n, dx=dx, L=L, c=c, shots=shot_geo, kw=kw, counts=counts)
 
 
@@ -242,6 +245,47 @@ def gradient_error_vs_fp64(wl, shots=8):
     return {"rel_l2": err, "sample": f"{shots} shots, full grid and {op.n_steps} steps"}
 
 
+def misfit_bench(rows, n_t, dt_obs, f_max, dev, reps=5):
+    """wave_fwi's band-limited misfit (BandLimitedMSE) and its adjoint source on traces of the
+    workload's shape, timed with the library's HIP events on the filter launches."""
+    from types import SimpleNamespace
+
+    import torch
+
+    from nefi_amd import _native as nat
+    from nefi_amd.instances.wave_fwi import BandLimitedMSE
+
+    g = torch.Generator(dev).manual_seed(3)
+    pred = torch.randn(rows, n_t, device=dev, generator=g).requires_grad_(True)
+    obs = torch.randn(rows, n_t, device=dev, generator=g)
+    ctx = SimpleNamespace(pred=pred, obs=SimpleNamespace(data=obs, mask=None))
+    loss_fn = BandLimitedMSE(f_max, dt_obs)
+    for _ in range(2):
+        torch.autograd.grad(loss_fn(ctx), pred)
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.autograd.grad(loss_fn(ctx), pred)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    ms, launches = nat.profile_read(nat.K_FILTER)
+    nat.profile_enable(False)
+    # fused residual pass: read pred, obs, write Z; obs pass: read obs; adjoint: read Z, write
+    bytes_per_call = 24 * rows * n_t
+    per_call_s = ms / 1e3 / reps
+    return {
+        "workload": f"BandLimitedMSE(f_max={f_max:g} Hz) forward + adjoint source, {rows} traces "
+                    f"x {n_t} samples, float32",
+        "ms_per_call": round(per_call_s * 1e3, 3),
+        "wall_ms_per_call": round(wall * 1e3, 3),
+        "filter_launches_per_call": launches // reps,
+        "hbm_bytes_per_call": bytes_per_call,
+        "achieved_gbs": round(bytes_per_call / per_call_s / 1e9, 1),
+        "frac_of_hbm": round(bytes_per_call / per_call_s / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
 def measured_traffic(kernel_key: str):
     """HBM bytes per launch of one kernel instance from the committed rocprofv3 counter runs
     (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration kernel)."""
@@ -266,11 +310,15 @@ def main() -> int:
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
+    ap.add_argument("--absorbing", default="sponge", choices=("sponge", "pml", "cerjan"),
+                    help="absorbing layer of the survey (the benchmark workload uses the sponge)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autograd", action="store_true",
                     help="skip the plain-autograd comparator on the GPU")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the float32-vs-float64 gradient check")
+    ap.add_argument("--no-misfit", action="store_true",
+                    help="skip timing the band-limited misfit on the workload's traces")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 512 MiB read + 512 MiB write elementwise kernel "
                          "(reference point for HBM counters)")
@@ -295,7 +343,7 @@ def main() -> int:
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    wl = survey(args.n, args.shots, args.nt)
+    wl = survey(args.n, args.shots, args.nt, absorbing=args.absorbing)
     n = wl["n"]
     dom = Domain((n, n), ((0.0, wl["L"]), (0.0, wl["L"])))
     op = ShotBatchWaveOperator(dom, wl["shots"], grad_mode="checkpoint",
@@ -369,18 +417,24 @@ def main() -> int:
              "steps_blocked": "k4_steps",
              "adjoint": f"{fam}_adjoint" if rows_t else "k_adjoint",
              "adjoint_blocked": "k5_adjoint"}[dom_kind]
-    kname += f"<order 4, sponge{', rows %d' % rows_t if rows_t else ''}"
+    pml = op.absorbing == "pml"
+    if pml and rows_t:  # PML plans always run the LDS-tile face kernels, 8 or 16 rows
+        kname = {"step": "k3_step_pml", "adjoint": "k3_adjoint_pml"}[dom_kind]
+        rows_t = 8 if rows_t <= 8 else 16
+    kname += f"<order 4, {op.absorbing}{', rows %d' % rows_t if rows_t else ''}"
     kname += f", depth {depth}>" if blocked else ">"
     tot_ms, launches = timing[dom_kind]
     # (under a counter-collecting profiler the library's events record nothing)
     avg_s = tot_ms / 1e3 / launches if launches and tot_ms > 0 else float("nan")
     steps_per_launch = units[dom_kind] / launches if launches else 1.0
     cells_per_launch = padded[0] * padded[1] * args.shots * steps_per_launch
-    achieved = BYTES_PER_UPDATE[dom_kind] * cells_per_launch / avg_s / 1e9 if avg_s == avg_s else 0.0
+    bpu = (BYTES_PER_UPDATE_PML if pml else BYTES_PER_UPDATE)[dom_kind]
+    achieved = bpu * cells_per_launch / avg_s / 1e9 if avg_s == avg_s else 0.0
     r_code = 2 if op.order == 4 else 1
     a_code = {"none": 0, "sponge": 1, "cerjan": 2, "pml": 3}[op.absorbing]
-    inst = f"nw::{kname.split('<')[0]}<{r_code}, {a_code}, {rows_t}"
-    if fam == "k2" and dom_kind == "step":
+    inst = (f"nw::{kname.split('<')[0]}<{r_code}, {rows_t}" if pml and rows_t else
+            f"nw::{kname.split('<')[0]}<{r_code}, {a_code}, {rows_t}")
+    if fam == "k2" and dom_kind == "step" and not pml:
         inst += f", {tiles['rows_per_iter']}"
     if blocked:
         inst += f", {depth}"
@@ -394,13 +448,17 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
         "traffic_source": traffic_src,
-        "bytes_per_cell_update": BYTES_PER_UPDATE[dom_kind],
+        "bytes_per_cell_update": bpu,
         "steps_per_launch": round(steps_per_launch, 3),
         "avg_launch_us": round(avg_s * 1e6, 2),
     }
     kernel_ms = {k: {"total_ms": round(v[0] / args.steps, 2), "launches": v[1] // args.steps,
                      "steps": units[k] // args.steps}
                  for k, v in timing.items()}
+    misfit = None
+    if not args.no_misfit:
+        misfit = misfit_bench(rows, op.n_t, op.dt_obs, 0.5 * wl["kw"]["f0"], dev)
+        print(f"[bench] misfit: {misfit}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
     comp = None
     if not args.no_autograd:
@@ -429,7 +487,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"2-D FWI gradient, {n}^2 grid (+20-cell sponge = {padded[0]}^2), "
+            "workload": f"2-D FWI gradient, {n}^2 grid (+20-cell {op.absorbing} = {padded[0]}^2), "
                         f"{args.shots} shots, src depth U[2,40] cells, receivers U[64,512]/shot, "
                         f"{op.n_steps} steps, snapshot every {args.ckpt}",
             "grid": list(padded),
@@ -444,6 +502,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "autograd_gpu": comp,
         "gradient_error_vs_fp64": grad_err,
+        "misfit": misfit,
     }
     print(json.dumps(line), flush=True)
     return 0

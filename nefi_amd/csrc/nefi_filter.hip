@@ -43,8 +43,7 @@ namespace {
 constexpr int kFilterThreads = 256;
 constexpr int kFilterBlocks = 2048;  // workgroups of one apply (a loop over row pairs)
 constexpr int kMinLog2 = 6;
-constexpr int kMaxLog2F32 = 14;  // 16384 float2 = 128 KiB of LDS
-constexpr int kMaxLog2F64 = 13;  // 8192 double2 = 128 KiB
+// FFT lengths: 16384 float2 (+1/8 padding: 144 KiB of LDS) or 8192 double2 (144 KiB)
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
 template <typename T>
@@ -112,11 +111,138 @@ __device__ __forceinline__ void block_sums(double v, double m, double* out) {
   }
 }
 
+// LDS slot of sequence element e: one pad slot after every 8 elements, so the stride-8 group
+// accesses of the passes with small spans fall on different banks
+__device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
+
+// exp(-pi i m / 2^k) for the offsets m < 2^k <= 4 of a group stage (constants after unrolling)
+template <typename V>
+__device__ __forceinline__ V unit_root(int k, int m) {
+  constexpr double c = 0.70710678118654752440;
+  V r;
+  r.x = 1;
+  r.y = 0;
+  if (k == 1 && m == 1) {
+    r.x = 0;
+    r.y = -1;
+  } else if (k == 2 && m == 1) {
+    r.x = c;
+    r.y = -c;
+  } else if (k == 2 && m == 2) {
+    r.x = 0;
+    r.y = -1;
+  } else if (k == 2 && m == 3) {
+    r.x = -c;
+    r.y = -c;
+  }
+  return r;
+}
+
+// R consecutive radix-2 stages done in registers. The stages of spans 2^lh_top down to
+// s = 2^(lh_top - R + 1) (forward) or s up to 2^lh_top (inverse) only combine the 2^R elements
+// base + q s, q < 2^R, of a group (base = block of 2^(lh_top + 1) elements + offset < s), so a
+// thread loads its groups, runs the R stages and stores them: one LDS round trip and one
+// barrier per R stages. Forward stages are decimation in frequency ((a + b, (a - b) w)),
+// inverse stages decimation in time ((a + b conj(w), a - b conj(w))), with
+// w = exp(-2 pi i j / (2h)) for the element's offset j in its half-block of span h. In a group
+// j = off + m s (m < h / s), so w is the table's exp(-2 pi i off / (2h)) times the constant
+// exp(-pi i m s / h): one twiddle load per stage and group.
+template <typename T, int LOG2L, int R, bool INV>
+__device__ __forceinline__ void fft_group_pass(typename Cplx<T>::type* z,
+                                               const typename Cplx<T>::type* tw, int lh_top) {
+  using V = typename Cplx<T>::type;
+  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q;
+  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;  // groups per thread
+  constexpr int CH0 = sizeof(T) == 4 ? 4 : 2;
+  constexpr int CH = KG < CH0 ? KG : CH0;  // groups loaded before the first store
+  const int ls = lh_top - (R - 1);
+  const int s = 1 << ls;
+#pragma unroll
+  for (int c0 = 0; c0 < KG; c0 += CH) {
+    V e[CH][Q];
+    int base[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = threadIdx.x + kFilterThreads * (c0 + c);
+      base[c] = -1;
+      if (G % kFilterThreads == 0 || g < G) {
+        const int off = g & (s - 1);
+        base[c] = ((g >> ls) << (ls + R)) + off;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) e[c][q] = z[pad8(base[c] + q * s)];
+      }
+    }
+    V tb[CH][R];  // exp(-2 pi i off / (2h)) for the spans h = s 2^k of the group's stages
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (base[c] < 0) continue;
+      const int off = base[c] & (s - 1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) tb[c][k] = tw[off << (LOG2L - ls - k - 1)];
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (base[c] < 0) continue;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        const int k = INV ? t : R - 1 - t;  // the stage's span is s 2^k
+        const int d = 1 << k;               // element distance in q
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (q & d) continue;
+          const int m = q & (d - 1);
+          const V w = m == 0 ? tb[c][k] : cmul(tb[c][k], unit_root<V>(k, m));
+          const V a = e[c][q];
+          const V b = INV ? cmul_conj(e[c][q + d], w) : e[c][q + d];
+          V sum, dif;
+          sum.x = a.x + b.x;
+          sum.y = a.y + b.y;
+          dif.x = a.x - b.x;
+          dif.y = a.y - b.y;
+          e[c][q] = sum;
+          e[c][q + d] = INV ? dif : cmul(dif, w);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) z[pad8(base[c] + q * s)] = e[c][q];
+    }
+  }
+}
+
+// forward transform (natural order in, bit-reversed out): passes of 3 stages from the top
+// span down, then the remaining 1 or 2; the inverse runs the same groups in reverse order
+template <typename T, int LOG2L, bool INV>
+__device__ __forceinline__ void fft_all(typename Cplx<T>::type* z,
+                                        const typename Cplx<T>::type* tw) {
+  constexpr int P = LOG2L / 3, REM = LOG2L % 3;
+  if constexpr (!INV) {
+#pragma unroll 1
+    for (int q = 0; q < P; ++q) {
+      fft_group_pass<T, LOG2L, 3, false>(z, tw, LOG2L - 1 - 3 * q);
+      __syncthreads();
+    }
+    if constexpr (REM > 0) {
+      fft_group_pass<T, LOG2L, (REM > 0 ? REM : 1), false>(z, tw, REM - 1);
+      __syncthreads();
+    }
+  } else {
+    if constexpr (REM > 0) {
+      fft_group_pass<T, LOG2L, (REM > 0 ? REM : 1), true>(z, tw, REM - 1);
+      __syncthreads();
+    }
+#pragma unroll 1
+    for (int q = P - 1; q >= 0; --q) {
+      fft_group_pass<T, LOG2L, 3, true>(z, tw, LOG2L - 1 - 3 * q);
+      __syncthreads();
+    }
+  }
+}
+
 template <typename T, int LOG2L>
 __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
   using V = typename Cplx<T>::type;
   constexpr int L = 1 << LOG2L;
-  __shared__ V z[L];
+  __shared__ V z[L + L / 8];  // padded (pad8)
   const int n = (int)io.n;
   double acc_v = 0.0, acc_m = 0.0;
   const int64_t pairs = (io.rows + 1) / 2;
@@ -138,56 +264,21 @@ __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
       V v;
       v.x = a;
       v.y = b;
-      z[t] = v;
+      z[pad8(t)] = v;
     }
     __syncthreads();
-    // forward transform, decimation in frequency: natural order in, bit-reversed out
-#pragma unroll 1
-    for (int lh = LOG2L - 1; lh >= 0; --lh) {
-      const int h = 1 << lh, ts = L >> (lh + 1);
-      for (int b = threadIdx.x; b < L / 2; b += kFilterThreads) {
-        const int j = b & (h - 1);
-        const int i = 2 * b - j;
-        const V u = z[i], v = z[i + h];
-        V s, d;
-        s.x = u.x + v.x;
-        s.y = u.y + v.y;
-        d.x = u.x - v.x;
-        d.y = u.y - v.y;
-        z[i] = s;
-        z[i + h] = cmul(d, io.tw[j * ts]);
-      }
-      __syncthreads();
-    }
+    fft_all<T, LOG2L, false>(z, io.tw);  // spectrum in bit-reversed order
     for (int t = threadIdx.x; t < L; t += kFilterThreads) {
       const T g = io.spec[t];
-      V v = z[t];
+      V v = z[pad8(t)];
       v.x = v.x * g;
       v.y = v.y * g;
-      z[t] = v;
+      z[pad8(t)] = v;
     }
     __syncthreads();
-    // inverse transform, decimation in time: bit-reversed in, natural order out
-#pragma unroll 1
-    for (int lh = 0; lh < LOG2L; ++lh) {
-      const int h = 1 << lh, ts = L >> (lh + 1);
-      for (int b = threadIdx.x; b < L / 2; b += kFilterThreads) {
-        const int j = b & (h - 1);
-        const int i = 2 * b - j;
-        const V u = z[i];
-        const V v = cmul_conj(z[i + h], io.tw[j * ts]);
-        V s, d;
-        s.x = u.x + v.x;
-        s.y = u.y + v.y;
-        d.x = u.x - v.x;
-        d.y = u.y - v.y;
-        z[i] = s;
-        z[i + h] = d;
-      }
-      __syncthreads();
-    }
+    fft_all<T, LOG2L, true>(z, io.tw);  // back to natural order
     for (int t = threadIdx.x; t < n; t += kFilterThreads) {
-      const V v = z[t];
+      const V v = z[pad8(t)];
       const T m0 = io.mask ? io.mask[o0 + t] : T(1);
       if (io.y) io.y[o0 + t] = io.scale * (m0 * v.x);
       acc_v += (double)m0 * ((double)v.x * (double)v.x);
