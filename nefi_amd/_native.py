@@ -1,5 +1,5 @@
-"""ctypes binding of ``libnefi_wave.so`` (declared in ``include/nefi_wave.h`` and
-``include/nefi_filter.h``).
+"""ctypes binding of ``libnefi_wave.so`` (declared in ``include/nefi_wave.h``,
+``include/nefi_filter.h`` and ``include/nefi_stepper.h``).
 
 The library is loaded after ``torch`` so that it binds to the HIP runtime torch already loaded
 (both carry the soname ``libamdhip64.so.7``); torch's streams and device pointers are then valid
@@ -59,6 +59,29 @@ FILTER_EXPORTS = (
     "nw_filter_length",
     "nw_filter_apply",
 )
+# every symbol include/nefi_stepper.h declares
+STEPPER_EXPORTS = (
+    "nw_ts_laplacian",
+    "nw_ts_wave_step",
+    "nw_ts_mur",
+    "nw_ts_mur_adjoint",
+    "nw_ts_wave_step_adjoint",
+)
+NW_TS_PERIODIC, NW_TS_NEUMANN, NW_TS_DIRICHLET, NW_TS_ABSORBING = 0, 1, 2, 3
+
+
+class NwTsGrid(ctypes.Structure):
+    """Mirror of ``struct nw_ts_grid`` (include/nefi_stepper.h)."""
+
+    _fields_ = [
+        ("ndim", c_int32),
+        ("dtype", c_int32),
+        ("boundary", c_int32),
+        ("reserved", c_int32),
+        ("batch", c_int64),
+        ("shape", c_int64 * 3),
+        ("h", c_double * 3),
+    ]
 
 
 class NwDesc(ctypes.Structure):
@@ -141,6 +164,12 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_filter_destroy": (i32, [vp]),
         "nw_filter_length": (i64, [vp]),
         "nw_filter_apply": (i32, [vp, vp, vp, vp, vp, i64, i32, c_double, vp, vp]),
+        "nw_ts_laplacian": (i32, [POINTER(NwTsGrid), vp, vp, vp]),
+        "nw_ts_wave_step": (i32, [POINTER(NwTsGrid), c_double, vp, vp, vp, vp, vp, vp, i32, vp]),
+        "nw_ts_mur": (i32, [POINTER(NwTsGrid), i32, c_double, vp, vp, vp, vp]),
+        "nw_ts_mur_adjoint": (i32, [POINTER(NwTsGrid), i32, c_double, vp, vp, vp, vp, vp, vp, vp]),
+        "nw_ts_wave_step_adjoint": (i32, [POINTER(NwTsGrid), c_double, vp, vp, vp, vp, vp, vp, vp,
+                                          vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

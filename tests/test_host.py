@@ -41,9 +41,10 @@ def test_library_loads_and_exports_every_declared_symbol():
 
     assert declared_in("nefi_wave.h") == set(nat.EXPORTS)
     assert declared_in("nefi_filter.h") == set(nat.FILTER_EXPORTS)
-    assert sorted(p.name for p in (ROOT / "include").glob("*.h")) == ["nefi_filter.h",
-                                                                     "nefi_wave.h"]
-    declared = set(nat.EXPORTS) | set(nat.FILTER_EXPORTS)
+    assert declared_in("nefi_stepper.h") == set(nat.STEPPER_EXPORTS)
+    assert sorted(p.name for p in (ROOT / "include").glob("*.h")) == [
+        "nefi_filter.h", "nefi_stepper.h", "nefi_wave.h"]
+    declared = set(nat.EXPORTS) | set(nat.FILTER_EXPORTS) | set(nat.STEPPER_EXPORTS)
     lib = nat.lib()
     for name in declared:
         assert getattr(lib, name) is not None
