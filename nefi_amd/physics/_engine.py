@@ -15,6 +15,7 @@ from __future__ import annotations
 import contextlib
 import ctypes
 import math
+import os
 import weakref
 from dataclasses import dataclass, field
 
@@ -348,6 +349,13 @@ class Workspace:
         self.plan = plan
         self.bufs: dict[str, torch.Tensor] = {}
         self.busy = False
+        self._side: torch.cuda.Stream | None = None
+
+    def side_stream(self) -> torch.cuda.Stream:
+        """A second stream (default priority) for work that overlaps the caller's stream."""
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.plan.device)
+        return self._side
 
     def get(self, name: str, numel: int, dtype=None, zero: bool = False) -> torch.Tensor:
         dt = dtype or self.plan.dtype
@@ -434,6 +442,10 @@ class Propagator:
         lam, psib = ws.state("lam")
         grad_m = ws.get("grad_m", P.grad_slices() * h.cells, torch.float64, zero=True)
         state, psi = ws.state("s")
+        if nb > 2 and self._overlap_fits(ws, K):
+            self._backward_overlapped(m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m,
+                                      state, psi)
+            nb = 0  # done
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
             if b < nb - 1:
@@ -454,6 +466,61 @@ class Propagator:
         )
         ws.release()
         return grad_c
+
+    def _overlap_fits(self, ws: Workspace, K: int) -> bool:
+        """Overlap needs a second keep buffer in free device memory (``NEFI_AMD_OVERLAP=0``
+        turns it off)."""
+        P = self.plan
+        if P.device.type != "cuda" or os.environ.get("NEFI_AMD_OVERLAP", "1") == "0":
+            return False
+        if "keep2" in ws.bufs and ws.bufs["keep2"].numel() >= (K + 1) * P.keep_slot:
+            return True
+        free, _ = torch.cuda.mem_get_info(P.device)
+        return (K + 1) * P.keep_slot * P.esize < 0.8 * free
+
+    def _backward_overlapped(self, m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m, state,
+                             psi) -> None:
+        """The reverse sweep with block b-1 recomputed on a second stream while the reverse
+        steps of block b run: two keep buffers alternate between blocks; a recompute waits
+        until the reverse steps that read its buffer have finished, and the reverse steps of a
+        block wait for its recompute."""
+        P = self.plan
+        N = P.host.n_steps
+        bufs = (keep, ws.get("keep2", (K + 1) * P.keep_slot))
+        main = torch.cuda.current_stream(P.device)
+        side = ws.side_stream()
+
+        def keep_of(b):
+            return bufs[(nb - 1 - b) % 2]
+
+        def recompute(b, after: torch.cuda.Event) -> torch.cuda.Event:
+            n0, n1 = b * K, min(N, (b + 1) * K)
+            with torch.cuda.stream(side):
+                side.wait_event(after)
+                prev, cur, faces = P.ckpt_views(ckpt, b)
+                state[0].copy_(prev)
+                state[1].copy_(cur)
+                for a, f in enumerate(faces):
+                    psi[a].copy_(f)
+                n_end = n1 if P.pml or P.option(nat.OPT_KEEP_LAP) else max(n0, n1 - 1)
+                P.forward(m, state, psi, n0, n_end, keep=keep_of(b))
+                done = torch.cuda.Event()
+                done.record(side)
+            return done
+
+        start = torch.cuda.Event()
+        start.record(main)  # the forward sweep's snapshots and last block are written
+        ready = {nb - 2: recompute(nb - 2, start)}
+        for b in range(nb - 1, -1, -1):
+            n0, n1 = b * K, min(N, (b + 1) * K)
+            if b < nb - 1:
+                main.wait_event(ready.pop(b))
+            P.adjoint(ma, lam, psib, n0, n1, keep=keep_of(b), residual=res,
+                      inject_end=(b == nb - 1), grad=grad_m)
+            if b >= 2:  # block b-2 reuses block b's keep buffer
+                read = torch.cuda.Event()
+                read.record(main)
+                ready[b - 2] = recompute(b - 2, read)
 
     # ---- PAT: p0 -> traces -----------------------------------------------------------------
     def ic_forward(self, m: torch.Tensor, p0: torch.Tensor) -> torch.Tensor:
