@@ -422,11 +422,19 @@ class Propagator:
         nb = -(-N // K)
         last0 = (nb - 1) * K
         ckpt = ws.get("ckpt", (nb - 1) * P.ckpt_slot) if nb > 1 else None
-        if last0 > 0:
-            P.forward(m, state, psi, 0, last0, traces=traces, ckpt=ckpt, ckpt_every=K)
+        # with room for the overlapped reverse sweep's second keep buffer, this sweep keeps the
+        # last two blocks, so the reverse sweep recomputes one block fewer
+        two = nb > 2 and self._overlap_fits(ws, K)
+        kept0 = last0 - K if two else last0
+        if kept0 > 0:
+            P.forward(m, state, psi, 0, kept0, traces=traces, ckpt=ckpt, ckpt_every=K)
+        if two:
+            P.forward(m, state, psi, kept0, last0, traces=traces,
+                      keep=ws.get("keep2", (K + 1) * P.keep_slot))
         keep = ws.get("keep", (K + 1) * P.keep_slot)
         P.forward(m, state, psi, last0, N, traces=traces, keep=keep)
-        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, _Lease(ws) if ws is self.ws else None)
+        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, two,
+                                _Lease(ws) if ws is self.ws else None)
 
     def fwi_backward(self, c: torch.Tensor, saved, residual: torch.Tensor) -> torch.Tensor:
         P = self.plan
@@ -435,7 +443,7 @@ class Propagator:
         grad_c = P.zeros(nf)
         if saved is None:
             return grad_c
-        m, ma, ckpt, keep, K, nb, ws, _lease = saved
+        m, ma, ckpt, keep, K, nb, ws, kept2, _lease = saved
         N = h.n_steps
         res = ws.get("residual", residual.numel())
         res.copy_(residual.reshape(-1))
@@ -444,7 +452,7 @@ class Propagator:
         state, psi = ws.state("s")
         if nb > 2 and self._overlap_fits(ws, K):
             self._backward_overlapped(m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m,
-                                      state, psi)
+                                      state, psi, kept2)
             nb = 0  # done
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
@@ -479,11 +487,12 @@ class Propagator:
         return (K + 1) * P.keep_slot * P.esize < 0.8 * free
 
     def _backward_overlapped(self, m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m, state,
-                             psi) -> None:
+                             psi, kept2: bool) -> None:
         """The reverse sweep with block b-1 recomputed on a second stream while the reverse
         steps of block b run: two keep buffers alternate between blocks; a recompute waits
         until the reverse steps that read its buffer have finished, and the reverse steps of a
-        block wait for its recompute."""
+        block wait for its recompute. ``kept2``: the forward sweep already filled the second
+        buffer with block nb-2."""
         P = self.plan
         N = P.host.n_steps
         bufs = (keep, ws.get("keep2", (K + 1) * P.keep_slot))
@@ -508,12 +517,14 @@ class Propagator:
                 done.record(side)
             return done
 
-        start = torch.cuda.Event()
-        start.record(main)  # the forward sweep's snapshots and last block are written
-        ready = {nb - 2: recompute(nb - 2, start)}
+        ready = {}
+        if not kept2:
+            start = torch.cuda.Event()
+            start.record(main)  # the forward sweep's snapshots and last block are written
+            ready[nb - 2] = recompute(nb - 2, start)
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
-            if b < nb - 1:
+            if b in ready:
                 main.wait_event(ready.pop(b))
             P.adjoint(ma, lam, psib, n0, n1, keep=keep_of(b), residual=res,
                       inject_end=(b == nb - 1), grad=grad_m)
