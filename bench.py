@@ -290,16 +290,17 @@ def misfit_bench(rows, n_t, dt_obs, f_max, dev, reps=5):
     }
 
 
-def measured_traffic(kernel_key: str):
-    """HBM bytes per launch of one kernel instance from the committed rocprofv3 counter runs
-    (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration kernel)."""
+def measured_traffic(kernel_key: str, xcd_map: int):
+    """HBM bytes per launch of one kernel instance (with the tile order ``xcd_map``) from the
+    committed rocprofv3 counter runs (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE
+    scaled by the calibration kernel)."""
     for path in sorted((ROOT / "profiles").glob("pmc_traffic_*.md")):
         text = path.read_text()
         start = text.rfind("```json")
         if start < 0:
             continue
         blob = json.loads(text[start + 7 : text.index("```", start + 7)])
-        k = blob["kernels"].get(kernel_key)
+        k = blob["kernels"].get(f"{kernel_key}|xcd{xcd_map}")
         if k:
             return k["fetch_bytes"] + k["write_bytes"], path.name
     return None, None
@@ -451,7 +452,7 @@ def main() -> int:
         inst += f", {tiles['rows_per_iter']}"
     if blocked:
         inst += f", {depth}"
-    traffic, traffic_src = measured_traffic(inst + ">")
+    traffic, traffic_src = measured_traffic(inst + ">", tiles["xcd_map"])
     roofline = {
         "bound": "hbm",
         "kernel": kname,
