@@ -44,8 +44,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"step": 16.0, "steps_blocked": 16.0, "adjoint": 28.0,
                     "adjoint_blocked": 28.0}
 # PML plans (--absorbing pml): the step also reads and writes both face fields (28 B); the
-# reverse step reads the adjoint face fields and both psi levels of the imaging term (52 B)
+# reverse step reads the adjoint face fields and both psi levels of the imaging term (52 B), or
+# with kept Laplacians (the PML default) one kept value instead of p^n and the psi levels (40 B)
 BYTES_PER_UPDATE_PML = {"step": 28.0, "adjoint": 52.0}
+BYTES_PER_UPDATE_PML_KEPT_LAP = {"step": 28.0, "adjoint": 40.0}
 
 
 def survey(n=1024, shots=64, n_t=4001, seed=0, absorbing="sponge"):
@@ -442,7 +444,11 @@ def main() -> int:
     avg_s = tot_ms / 1e3 / launches if launches and tot_ms > 0 else float("nan")
     steps_per_launch = units[dom_kind] / launches if launches else 1.0
     cells_per_launch = padded[0] * padded[1] * args.shots * steps_per_launch
-    bpu = (BYTES_PER_UPDATE_PML if pml else BYTES_PER_UPDATE)[dom_kind]
+    if pml:
+        bpu = (BYTES_PER_UPDATE_PML_KEPT_LAP if tiles["keep_laplacian"]
+               else BYTES_PER_UPDATE_PML)[dom_kind]
+    else:
+        bpu = BYTES_PER_UPDATE[dom_kind]
     achieved = bpu * cells_per_launch / avg_s / 1e9 if avg_s == avg_s else 0.0
     r_code = 2 if op.order == 4 else 1
     a_code = {"none": 0, "sponge": 1, "cerjan": 2, "pml": 3}[op.absorbing]

@@ -188,8 +188,10 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   kernel kinds run the fast 2-D kernels (reads back as 1 only then): the forward kernels
  *   write it as they step, and the reverse kernels read it pointwise for the imaging term
  *   (same values, no stencil). A keep sweep feeding such a reverse sweep over steps [n0, n1)
- *   must then run forward over [n0, n1) (not n1 - 1). 0 (default) keeps p^n: a blocked keep
- *   sweep then writes only the levels themselves, which measured faster overall.
+ *   must then run forward over [n0, n1) (not n1 - 1). 0 keeps p^n: a blocked keep sweep then
+ *   writes only the levels themselves, which measured faster overall. Default: 1 for PML
+ *   plans (the kept Laplacian replaces p^n and both face-field levels in the reverse step),
+ *   0 otherwise.
  * NW_OPT_TIME_BLOCK_ADJOINT: temporal blocking depth of the fast reverse step (1 default,
  *   2..4): up to that many reverse steps per launch, with the residual injections of recorded
  *   steps and the imaging sum inside the launch (lambda bit-identical to single steps; the

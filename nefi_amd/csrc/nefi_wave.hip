@@ -542,7 +542,7 @@ struct nw_plan {
   int tblock;
   float* tb_state[2];
   float* tapbuf;
-  int keep_lap_opt;  // NW_OPT_KEEP_LAP
+  int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
   // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth and the
   // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
   int tblock_adj;
@@ -758,10 +758,11 @@ static int hand_back(nw_plan* P, void** state, T* s0, T* s1, T* prev, T* cur, hi
 
 // Keep buffers hold the Laplacian of p^n (instead of p^n) when both fast 2-D kernel families
 // run: the forward kernels compute it anyway, and the reverse kernels' imaging term then reads
-// it pointwise instead of re-applying the stencil to a p tile.
+// it pointwise instead of re-applying the stencil to a p tile (PML: and instead of reading both
+// levels of both face fields, so there it is the default).
 static bool keep_lap(const nw_plan* P) {
-  return P->d.dtype == NW_F32 && P->d.absorbing != NW_PML && P->keep_lap_opt && P->rows[0] > 0 &&
-         P->rows[1] > 0;
+  const bool want = P->keep_lap_opt < 0 ? P->d.absorbing == NW_PML : P->keep_lap_opt != 0;
+  return P->d.dtype == NW_F32 && want && P->rows[0] > 0 && P->rows[1] > 0;
 }
 
 // The blocked sweep applies when the plan asks for it and the forward tiles have a blocked
@@ -1120,7 +1121,7 @@ static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** 
       else
         io.p_n = kslot(n);
       io.grad = grad;
-      if (pml) {
+      if (pml && !keep_lap(P)) {
         T* lo[3];
         T* hi[3];
         keep_psi(P, (T*)kslot(n), lo);
@@ -1316,7 +1317,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tblock = 1;
   p->tb_state[0] = p->tb_state[1] = nullptr;
   p->tapbuf = nullptr;
-  p->keep_lap_opt = 0;
+  p->keep_lap_opt = -1;
   p->tblock_adj = 1;
   p->injv = nullptr;
   p->graph_mode = 0;

@@ -1372,6 +1372,8 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
 #pragma unroll
         for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
         if (last_col) nx_row[4] = nx[4];
+        // the Laplacian with both face divergences: the reverse step's imaging operand
+        if (io.lap_out) st4(io.lap_out + base + (y * P2 + x), lap);
         if (io.ck_psi[1]) {
           st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
           if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
@@ -1438,7 +1440,9 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
   const float4 zcx = act ? ld4(g.zc[2] + x) : f4(0.f);
   const PmlX px = pml_x(g, x, act);
   const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
-  const bool imaging = io.p_n != nullptr && io.grad != nullptr;
+  const bool kept_lap = io.lap_n != nullptr;  // the forward step's Laplacian, faces included
+  const bool imaging = (io.p_n != nullptr || kept_lap) && io.grad != nullptr;
+  const bool p_tile = imaging && !kept_lap;
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1459,7 +1463,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
       for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
       tile_store<TY, R>(lq, vl);
     }
-    if (imaging) {
+    if (p_tile) {
       float4 vp[S::per_thread];
       tile_load<TY, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
       tile_store<TY, R>(lp, vp);
@@ -1469,13 +1473,13 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
       qw[k] = lrow(lq, r0 + k, lane);
-      pw[k] = imaging ? lrow(lp, r0 + k, lane) : f4(0.f);
+      pw[k] = p_tile ? lrow(lp, r0 + k, lane) : f4(0.f);
     }
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
       qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
-      pw[2 * R] = imaging ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
+      pw[2 * R] = p_tile ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
       if (act && y < D1) {
         const int ro = y * P2 + x;
         const float4 qn = lnb(lq, r0 + k + R, lane);
@@ -1495,7 +1499,9 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
         NW_PML_ADJ(x) NW_PML_ADJ(y) NW_PML_ADJ(z) NW_PML_ADJ(w)
 #undef NW_PML_ADJ
         float4 lapp = f4(0.f);
-        if (imaging) {
+        if (kept_lap && imaging) {
+          lapp = ld4(io.lap_n + base + ro);
+        } else if (p_tile) {
           const float4 pn = lnb(lp, r0 + k + R, lane);
           lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
         }
@@ -1521,7 +1527,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
 #undef NW_PML_ADJ_Y
           st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
           if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
-          if (imaging) {
+          if (p_tile) {
             const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
             const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
             const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
@@ -1553,7 +1559,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
 #pragma unroll
           for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
           if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
-          if (imaging) {
+          if (p_tile) {
             const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
             const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
             float hw[5];

@@ -167,6 +167,34 @@ def test_kept_laplacian_matches_kept_wavefield(absorbing, order):
         assert np.array_equal(y, y0) and np.array_equal(g, g0)
 
 
+@pytest.mark.parametrize("order", [4, 2])
+def test_pml_kept_laplacian_matches_kept_wavefield_and_face_fields(order):
+    """PML: keep buffers holding the forward step's Laplacian (face divergences included) give
+    exactly the gradient of keep buffers holding p^n and both face-field levels, which the
+    reverse step otherwise reads to rebuild that Laplacian; kept Laplacians are the PML
+    default."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey("pml", order, n=(70, 600))
+    print(f"[case pml kept-laplacian o{order}]", flush=True)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(17))
+    x = torch.tensor(c, dtype=torch.float32, device=DEV)
+    op({"c": x})
+    assert next(iter(op._cache.values())).plan.option(nat.OPT_KEEP_LAP) == 1  # the default
+    for tiles, lds in (([(16, 2), (16, 2)], (1, 1)), ([(8, 1), (8, 3)], (1, 1)),
+                       ([(32, 5), (4, 5)], (1, 0))):
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=lds, keep_lap=0)
+        y, g = _grad(op, c, w, tiles=tiles, lds=lds, keep_lap=1)
+        used = next(iter(op._cache.values())).plan.option(nat.OPT_KEEP_LAP)
+        print(f"pml o{order} tiles {tiles}: kept laplacian {used}, traces {rel(y, y0):.1e}, "
+              f"gradient {rel(g, g0):.1e}", flush=True)
+        assert used == 1
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
+
+
 @pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2),
                                              ("sponge", 2)])
 def test_blocked_reverse_matches_single_steps(absorbing, order):
