@@ -292,6 +292,71 @@ def misfit_bench(rows, n_t, dt_obs, f_max, dev, reps=5):
     }
 
 
+def pat3d_bench(args, dev) -> dict:
+    """The 3-D photoacoustic operator (192^3 + 10-cell sponge = 212^3, order 2, 4096 detectors on
+    the top face, 1500 steps, speed 1500 m/s +-5 % smooth) forward + adjoint in float32, on the
+    generic 3-D kernels: throughput, and the roofline of its dominant kernel from HIP events."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    from nefi_amd import _native as nat
+    from nefi_amd.domain import Domain
+    from nefi_amd.physics.wave import WaveInitialConditionOperator
+
+    n, dx = 192, 1e-4
+    L = n * dx
+    rng = np.random.default_rng(2)
+    pert = gaussian_filter(rng.standard_normal((n, n, n)), 6.0, mode="wrap")
+    c = 1500.0 * (1.0 + 0.05 * pert / np.abs(pert).max())
+    det = [[(i + 0.5) * L / 64, (j + 0.5) * L / 64, 0.5 * dx] for i in range(64)
+           for j in range(64)]
+    c_max = float(c.max())
+    op = WaveInitialConditionOperator(Domain((n, n, n), ((0.0, L),) * 3), det, n_t=1501,
+                                      dt_obs=0.3 * dx / c_max, c=torch.tensor(c), c_max=c_max,
+                                      order=2, absorbing="sponge", absorb_width=10 * dx)
+    p0 = torch.tensor(gaussian_filter(rng.standard_normal((n, n, n)), 3.0, mode="wrap"),
+                      dtype=torch.float32, device=dev, requires_grad=True)
+    w = torch.randn(len(det), 1501, device=dev, generator=torch.Generator(dev).manual_seed(3))
+
+    def step():
+        p0.grad = None
+        op({"p0": p0}).backward(w)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / args.steps
+    timing = {k: nat.profile_read(v) for k, v in (("step", nat.K_STEP), ("adjoint", nat.K_ADJOINT))}
+    nat.profile_enable(False)
+    padded = op.grid.padded_shape
+    cells = padded[0] * padded[1] * padded[2]
+    dom = max(timing, key=lambda k: timing[k][0])
+    ms, launches = timing[dom]
+    avg_s = ms / 1e3 / launches
+    bpu = 16.0  # step: p^n, p^{n-1}, m read, p^{n+1} written; reverse: lambda pair, m a, out
+    achieved = bpu * cells / avg_s / 1e9
+    return {
+        "metric": "3-D photoacoustic forward+adjoint Gcell-updates/s",
+        "value": round(cells * op.n_steps / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(elapsed * 1e3, 2),
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3-D PAT, 192^3 + 10-cell sponge = {padded[0]}^3, order 2, "
+                               f"4096 detectors, {op.n_steps} steps, forward + adjoint"},
+        "roofline": {"bound": "hbm", "kernel": f"k_{dom}<float, order 2, sponge> (generic)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell_update": bpu,
+                     "avg_launch_us": round(avg_s * 1e6, 2)},
+        "kernels_per_gradient": {k: {"total_ms": round(v[0] / args.steps, 2),
+                                     "launches": v[1] // args.steps} for k, v in timing.items()},
+    }
+
+
 def measured_traffic(kernel_key: str, xcd_map: int):
     """HBM bytes per launch of one kernel instance (with the tile order ``xcd_map``) from the
     committed rocprofv3 counter runs (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE
@@ -317,6 +382,9 @@ def main() -> int:
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
+    ap.add_argument("--workload", default="fwi", choices=("fwi", "pat3d"),
+                    help="fwi: the 64-shot 2-D gradient (the benchmark); pat3d: the 3-D "
+                         "photoacoustic forward + adjoint")
     ap.add_argument("--absorbing", default="sponge", choices=("sponge", "pml", "cerjan"),
                     help="absorbing layer of the survey (the benchmark workload uses the sponge)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -343,6 +411,9 @@ def main() -> int:
     from nefi_amd.physics.batched import ShotBatchWaveOperator
 
     dev = torch.device("cuda", 0)
+    if args.workload == "pat3d":
+        print(json.dumps(pat3d_bench(args, dev)), flush=True)
+        return 0
     if args.calibrate:
         a = torch.ones(1 << 27, device=dev)
         b = torch.empty_like(a)
