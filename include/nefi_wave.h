@@ -158,7 +158,7 @@ int32_t nw_forward(nw_plan* plan, const void* m, void** state, void** psi, int32
  * project's autotuner). kind 0 = forward step, 1 = reverse step. rows = rows of the strip one
  * wavefront marches down (4, 8, 16, 32 or 64; 0 selects the generic kernels),
  * shots_per_group = shots one workgroup advances in turn. Plans that the fast kernels cover
- * (2-D, float32, sponge/Cerjan/none, padded width a multiple of 4) start at rows = 16, 1 shot;
+ * (2-D, float32, padded width and row pitch multiples of 4) start at rows = 16, 1 shot;
  * others report rows = 0 and refuse nonzero rows. */
 int32_t nw_plan_set_tiles(nw_plan* plan, int32_t kind, int32_t rows, int32_t shots_per_group);
 int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
@@ -198,7 +198,11 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   float64 imaging sums are added in another order). Used when the reverse tiles have rows of
  *   8 or 16, the imaging term (if any) reads kept Laplacians (NW_OPT_KEEP_LAP) and a shot has
  *   at most 2048 injection cells; other calls step one by one. Shares the plan's blocking
- *   wavefields with NW_OPT_TIME_BLOCK. */
+ *   wavefields with NW_OPT_TIME_BLOCK.
+ * NW_OPT_FAST_3D: 1 (default where they cover the plan: 3-D, float32, sponge/Cerjan/none,
+ *   padded width and row pitch multiples of 4) runs the 2.5-D tiled 3-D kernels for forward
+ *   steps and for reverse steps without an imaging term; 0 the generic kernels. Plans they do
+ *   not cover report 0 and refuse 1. */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,
@@ -208,7 +212,8 @@ enum nw_option {
   NW_OPT_GRAPH_COUNT = 5,
   NW_OPT_TIME_BLOCK = 6,
   NW_OPT_KEEP_LAP = 7,
-  NW_OPT_TIME_BLOCK_ADJOINT = 8
+  NW_OPT_TIME_BLOCK_ADJOINT = 8,
+  NW_OPT_FAST_3D = 9
 };
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);

@@ -214,6 +214,7 @@ OPT_GRAPH_COUNT = 5
 OPT_TIME_BLOCK = 6
 OPT_KEEP_LAP = 7
 OPT_TIME_BLOCK_ADJOINT = 8
+OPT_FAST_3D = 9
 
 
 def profile_enable(on: bool) -> None:

@@ -543,6 +543,7 @@ struct nw_plan {
   float* tb_state[2];
   float* tapbuf;
   int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
+  int fast3d;        // NW_OPT_FAST_3D: the 2.5-D tiled 3-D float32 kernels (when they cover it)
   // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth and the
   // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
   int tblock_adj;
@@ -679,6 +680,10 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
                   P->xcd_map | (P->rows_per_iter == 2 ? 2 : 0) | (P->lds_tiles[0] ? 4 : 0), st);
       return;
     }
+    if (P->fast3d && !io.lap_out) {
+      fast3d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
+      return;
+    }
   }
   const int r = P->d.order == 4 ? 2 : 1;
   switch (P->d.absorbing * 2 + (r - 1)) {
@@ -699,6 +704,10 @@ static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
     if (P->rows[1] > 0) {
       fast2d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[1], P->spw[1],
                      P->xcd_map | (P->lds_tiles[1] ? 4 : 0), st);
+      return;
+    }
+    if (P->fast3d && !io.p_n && !io.lap_n) {  // no imaging term
+      fast3d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
       return;
     }
   }
@@ -1318,6 +1327,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tb_state[0] = p->tb_state[1] = nullptr;
   p->tapbuf = nullptr;
   p->keep_lap_opt = -1;
+  p->fast3d = fast3d_supported(p->gf, d.absorbing, d.dtype) ? 1 : 0;
   p->tblock_adj = 1;
   p->injv = nullptr;
   p->graph_mode = 0;
@@ -1410,6 +1420,12 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     P->keep_lap_opt = value != 0;
     return NW_OK;
   }
+  if (option == NW_OPT_FAST_3D) {
+    if (value && !fast3d_supported(P->gf, P->d.absorbing, P->d.dtype))
+      return fail(NW_ERR_UNSUPPORTED, "the fast 3-D kernels do not cover this plan");
+    P->fast3d = value != 0;
+    return NW_OK;
+  }
   if (option == NW_OPT_TIME_BLOCK || option == NW_OPT_TIME_BLOCK_ADJOINT) {
     if (value < 1 || value > kMaxDepth)
       return fail(NW_ERR_ARG, "time block depth must be 1.." + std::to_string(kMaxDepth));
@@ -1464,6 +1480,10 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_KEEP_LAP) {
     *value = keep_lap(P) ? 1 : 0;
+    return NW_OK;
+  }
+  if (option == NW_OPT_FAST_3D) {
+    *value = P->fast3d;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

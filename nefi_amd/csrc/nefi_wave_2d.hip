@@ -1606,11 +1606,286 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
 }
 
 // ------------------------------------------------------------------------------------------
+// 3-D steps: 2.5-D tiles marching along the slowest axis
+// ------------------------------------------------------------------------------------------
+// A workgroup owns TY rows x 256 columns of one shot's grid and a chunk of ZC planes of the
+// slow axis (axis 0). The planes of the stencil's input field sit in a ring of 2R+1 LDS planes
+// (each with R halo rows and 4 margin columns on either side); plane z is computed from the
+// ring while the loads of plane z+R+1 are in flight in registers, so each workgroup waits on
+// HBM only in its prologue. Each input plane is read once per workgroup plus the chunk's 2R
+// halo planes and the strip's 2R halo rows (both mostly from L2); the pointwise operands
+// (p^{n-1}, m; lambda^{n+1}, lambda^{n+2}) are read per cell. Per cell the arithmetic is the
+// 2-D tiles' with axis 0 in front: centre term, then the fused multiply-adds of axes 0, 1, 2;
+// damping from the three profile values summed (sponge) or minimised (Cerjan) in axis order.
+template <int TY, int R>
+struct Shape3 {
+  static constexpr int rows = TY + 2 * R;
+  static constexpr int n4 = rows * (kLW / 4);
+  static constexpr int per_thread = (n4 + 255) / 256;
+  static constexpr int my_rows = TY / 4;
+  static constexpr int ring = 2 * R + 1;
+  static constexpr int plane = rows * kLW;  // floats of one LDS plane
+};
+
+struct Tile3 {
+  int cx, ry, cz, s;
+  bool valid;
+};
+
+// tiles numbered column chunk fastest, then row strip, z chunk, shot; XCD-contiguous ranges
+// as in tile_of
+__device__ __forceinline__ Tile3 tile3_of(int nX, int nY, int nZ, int shots, int xcd_map) {
+  const int T = nX * nY * nZ * shots;
+  const int per = (T + kXcds - 1) / kXcds;
+  const int i = blockIdx.x;
+  const int t = xcd_map ? (i % kXcds) * per + i / kXcds : i;
+  Tile3 r;
+  r.valid = t < T;
+  r.cx = t % nX;
+  r.ry = (t / nX) % nY;
+  r.cz = (t / (nX * nY)) % nZ;
+  r.s = t / (nX * nY * nZ);
+  return r;
+}
+
+// damping of one cell from its three profile values
+template <int ABS>
+__device__ __forceinline__ void damp3(float hz, float hy, float hx, float& a, float& b,
+                                      float& tap) {
+  a = 1.f;
+  b = 1.f;
+  tap = 1.f;
+  if (ABS == NW_SPONGE) {
+    const float half = (hz + hy) + hx;
+    a = 1.f / (1.f + half);
+    b = 1.f - half;
+  } else if (ABS == NW_CERJAN) {
+    tap = hz < tap ? hz : tap;
+    tap = hy < tap ? hy : tap;
+    tap = hx < tap ? hx : tap;
+    a = tap;
+  }
+}
+
+template <int ABS>
+__device__ __forceinline__ Damp4<ABS> damp3_4(float hz, float hy, const float4& hx) {
+  Damp4<ABS> d;
+  damp3<ABS>(hz, hy, hx.x, d.a.x, d.b.x, d.tap.x);
+  damp3<ABS>(hz, hy, hx.y, d.a.y, d.b.y, d.tap.y);
+  damp3<ABS>(hz, hy, hx.z, d.a.z, d.b.z, d.tap.z);
+  damp3<ABS>(hz, hy, hx.w, d.a.w, d.b.w, d.tap.w);
+  return d;
+}
+
+// one cell of the 3-D cross stencil
+template <int R>
+__device__ __forceinline__ float stencil3(const Geo<float>& g, float c, float zm1, float zp1,
+                                          float zm2, float zp2, float ym1, float yp1, float ym2,
+                                          float yp2, float xm1, float xp1, float xm2, float xp2) {
+  float s = __builtin_fmaf(g.c1[0], zm1 + zp1, g.cc * c);
+  if (R == 2) s = __builtin_fmaf(g.c2[0], zm2 + zp2, s);
+  s = __builtin_fmaf(g.c1[1], ym1 + yp1, s);
+  if (R == 2) s = __builtin_fmaf(g.c2[1], ym2 + yp2, s);
+  s = __builtin_fmaf(g.c1[2], xm1 + xp1, s);
+  if (R == 2) s = __builtin_fmaf(g.c2[2], xm2 + xp2, s);
+  return s;
+}
+
+// stencil at the lane's four columns of LDS row r of the ring plane pz; the slow-axis
+// neighbours from the ring planes pm1, pp1 (and pm2, pp2 for R = 2) at the same row
+template <int R>
+__device__ __forceinline__ float4 lap3(const Geo<float>& g, const float* pz, const float* pm1,
+                                       const float* pp1, const float* pm2, const float* pp2,
+                                       int r, int lane) {
+  const float4 c = lrow(pz, r, lane);
+  const float4 zm1 = lrow(pm1, r, lane), zp1 = lrow(pp1, r, lane);
+  const float4 zm2 = R == 2 ? lrow(pm2, r, lane) : f4(0.f);
+  const float4 zp2 = R == 2 ? lrow(pp2, r, lane) : f4(0.f);
+  const float4 ym1 = lrow(pz, r - 1, lane), yp1 = lrow(pz, r + 1, lane);
+  const float4 ym2 = R == 2 ? lrow(pz, r - 2, lane) : f4(0.f);
+  const float4 yp2 = R == 2 ? lrow(pz, r + 2, lane) : f4(0.f);
+  const float4 nb = lnb(pz, r, lane);  // x-2, x-1, x+4, x+5
+  float4 o;
+  o.x = stencil3<R>(g, c.x, zm1.x, zp1.x, zm2.x, zp2.x, ym1.x, yp1.x, ym2.x, yp2.x, nb.y, c.y,
+                    nb.x, c.z);
+  o.y = stencil3<R>(g, c.y, zm1.y, zp1.y, zm2.y, zp2.y, ym1.y, yp1.y, ym2.y, yp2.y, c.x, c.z,
+                    nb.y, c.w);
+  o.z = stencil3<R>(g, c.z, zm1.z, zp1.z, zm2.z, zp2.z, ym1.z, yp1.z, ym2.z, yp2.z, c.y, c.w,
+                    c.x, nb.z);
+  o.w = stencil3<R>(g, c.w, zm1.w, zp1.w, zm2.w, zp2.w, ym1.w, yp1.w, ym2.w, yp2.w, c.z, nb.z,
+                    c.y, nb.w);
+  return o;
+}
+
+// Loads of plane z of a field's tile (rows y0-R .. y0+TY+R-1, columns x0-4 .. x0+259; zero
+// outside the grid), multiplied pointwise by the same cells of `mul` when it is given.
+template <int TY, int R>
+__device__ __forceinline__ void plane_fetch(const float* f, const float* mul, int z, int y0,
+                                            int x0, int D0, int D1, int D2, int P2,
+                                            int64_t plane, float4* v) {
+  using S = Shape3<TY, R>;
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+    const int gy = y0 - R + row, gx = x0 - 4 + 4 * c4;
+    const bool ok = i < S::n4 && z >= 0 && z < D0 && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
+    const int64_t off = (int64_t)z * plane + (gy * P2 + gx);
+    v[k] = ok ? ld4(f + off) : f4(0.f);
+    if (mul != nullptr && ok) v[k] = mul4(ld4(mul + off), v[k]);
+  }
+}
+
+template <int TY, int R>
+__device__ __forceinline__ void plane_put(float* lds, const float4* v) {
+  using S = Shape3<TY, R>;
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < S::n4) *reinterpret_cast<float4*>(lds + 4 * i) = v[k];
+  }
+}
+
+__device__ __forceinline__ int ring_slot(int z, int n) { return ((z % n) + n) % n; }
+
+// The ring march shared by both kernels: fills the ring with planes z0-R .. z0+R-1, then for
+// each plane z of the chunk puts plane z+R into the ring, starts the loads of plane z+R+1 and
+// calls cell(z, ring planes) once the ring holds z-R .. z+R.
+template <int TY, int R, typename F>
+__device__ __forceinline__ void ring_march(float* ring, const float* f, const float* mul,
+                                           int z0, int z1, int y0, int x0, int D0, int D1,
+                                           int D2, int P2, int64_t plane, F cell) {
+  using S = Shape3<TY, R>;
+  float4 v[S::per_thread];
+  for (int z = z0 - R; z < z0 + R; ++z) {
+    plane_fetch<TY, R>(f, mul, z, y0, x0, D0, D1, D2, P2, plane, v);
+    plane_put<TY, R>(ring + ring_slot(z, S::ring) * S::plane, v);
+  }
+  plane_fetch<TY, R>(f, mul, z0 + R, y0, x0, D0, D1, D2, P2, plane, v);
+  for (int z = z0; z < z1; ++z) {
+    __syncthreads();  // plane z-1 is computed: the slot of plane z-R-1 is free
+    plane_put<TY, R>(ring + ring_slot(z + R, S::ring) * S::plane, v);
+    __syncthreads();
+    if (z + 1 < z1) plane_fetch<TY, R>(f, mul, z + R + 1, y0, x0, D0, D1, D2, P2, plane, v);
+    const float* pz = ring + ring_slot(z, S::ring) * S::plane;
+    const float* pm1 = ring + ring_slot(z - 1, S::ring) * S::plane;
+    const float* pp1 = ring + ring_slot(z + 1, S::ring) * S::plane;
+    const float* pm2 = R == 2 ? ring + ring_slot(z - 2, S::ring) * S::plane : pz;
+    const float* pp2 = R == 2 ? ring + ring_slot(z + 2, S::ring) * S::plane : pz;
+    cell(z, pz, pm1, pp1, pm2, pp2);
+  }
+}
+
+template <int R, int ABS, int TY, int ZC>
+__global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, int shots,
+                                                int xcd_map) {
+  using S = Shape3<TY, R>;
+  __shared__ __align__(16) float ring[S::ring * S::plane];
+  const int D0 = (int)g.D[0], D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int64_t plane = (int64_t)D1 * P2;
+  const Tile3 tl = tile3_of(div_up(D2, 256), div_up(D1, TY), div_up(D0, ZC), shots, xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY, z0 = tl.cz * ZC, s = tl.s;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;
+  const bool act = x < D2;
+  const int64_t base = (int64_t)s * g.cells;
+  const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
+  ring_march<TY, R>(ring, io.p + base, nullptr, z0, min(D0, z0 + ZC), y0, x0, D0, D1, D2, P2,
+                    plane, [&](int z, const float* pz, const float* pm1, const float* pp1,
+                               const float* pm2, const float* pp2) {
+    const float hz = ABS != NW_NONE ? g.damp[0][z] : 0.f;
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      if (!act || y >= D1) continue;
+      const int rr = r0 + k + R;
+      const float4 lap = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
+      const int64_t ro = (int64_t)z * plane + (y * P2 + x);
+      const float4 pc = lrow(pz, rr, lane);
+      const float4 pp = ld4(io.p_prev + base + ro);
+      const float4 mm = ld4(io.m + ro);
+      const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+      const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
+      const int tpv = io.taper_prev;
+      float4 o = make_float4(rhs1<ABS>(pc.x, pp.x, mm.x, lap.x, dm.b.x, dm.tap.x, tpv),
+                             rhs1<ABS>(pc.y, pp.y, mm.y, lap.y, dm.b.y, dm.tap.y, tpv),
+                             rhs1<ABS>(pc.z, pp.z, mm.z, lap.z, dm.b.z, dm.tap.z, tpv),
+                             rhs1<ABS>(pc.w, pp.w, mm.w, lap.w, dm.b.w, dm.tap.w, tpv));
+      for (int t = 0; t < io.src_taps; ++t) {
+        const int64_t d = io.src_idx[s * io.src_taps + t] - ro;
+        if (d >= 0 && d < 4) {
+          const float sv = io.src_val[s * io.src_taps + t] * w_n;
+          o.x += d == 0 ? sv : 0.f;
+          o.y += d == 1 ? sv : 0.f;
+          o.z += d == 2 ? sv : 0.f;
+          o.w += d == 3 ? sv : 0.f;
+        }
+      }
+      if (ABS != NW_NONE) o = mul4(dm.a, o);
+      st4(io.p_next + base + ro, o);
+      if (io.ck_prev) {
+        st4(io.ck_prev + base + ro, pp);
+        st4(io.ck_cur + base + ro, pc);
+      }
+    }
+  });
+}
+
+// The reverse step without an imaging term (the initial-value operator's sweep): lambda^n from
+// the stencil of q = (m a) lambda^{n+1}, which the ring holds, and the pointwise lambda^{n+1},
+// lambda^{n+2}.
+template <int R, int ABS, int TY, int ZC>
+__global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io, int shots,
+                                                   int xcd_map) {
+  using S = Shape3<TY, R>;
+  __shared__ __align__(16) float ring[S::ring * S::plane];
+  const int D0 = (int)g.D[0], D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
+  const int64_t plane = (int64_t)D1 * P2;
+  const Tile3 tl = tile3_of(div_up(D2, 256), div_up(D1, TY), div_up(D0, ZC), shots, xcd_map);
+  if (!tl.valid) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY, z0 = tl.cz * ZC, s = tl.s;
+  const int x = x0 + 4 * lane;
+  const int r0 = wave * S::my_rows;
+  const bool act = x < D2;
+  const int64_t base = (int64_t)s * g.cells;
+  const float* L1 = io.lam1 + base;
+  const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
+  const int tpn = io.taper_next;
+  ring_march<TY, R>(ring, L1, io.ma, z0, min(D0, z0 + ZC), y0, x0, D0, D1, D2, P2, plane,
+                    [&](int z, const float* pz, const float* pm1, const float* pp1,
+                        const float* pm2, const float* pp2) {
+    const float hz = ABS != NW_NONE ? g.damp[0][z] : 0.f;
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      if (!act || y >= D1) continue;
+      const int rr = r0 + k + R;
+      const float4 lapq = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
+      const int64_t ro = (int64_t)z * plane + (y * P2 + x);
+      const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
+      const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+      const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
+      float4 rb;
+      const float4 o = make_float4(
+          adj1<ABS>(lc.x, l2c.x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
+          adj1<ABS>(lc.y, l2c.y, lapq.y, dm.a.y, dm.b.y, dm.tap.y, tpn, rb.y),
+          adj1<ABS>(lc.z, l2c.z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
+          adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
+      st4(io.lam0 + base + ro, o);
+    }
+  });
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 bool fast2d_supported(const Geo<float>& g, int absorbing, int dtype) {
   (void)absorbing;
-  return g.nd == 2 && dtype == NW_F32 && g.D[2] % 4 == 0;
+  return g.nd == 2 && dtype == NW_F32 && g.D[2] % 4 == 0 && g.P2 % 4 == 0;
 }
 
 // 1-D grid of kXcds * ceil(tiles / kXcds) workgroups (tile_of maps them, a few return)
@@ -1835,6 +2110,76 @@ void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const 
                   (unsigned)(levels - 1));
   hipLaunchKernelGGL(k_sample_levels, grid, dim3(256), 0, (hipStream_t)stream, g, io, tapbuf,
                      tap_stride, shots, lj);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// 3-D launchers
+// ------------------------------------------------------------------------------------------
+bool fast3d_supported(const Geo<float>& g, int absorbing, int dtype) {
+  return g.nd == 3 && dtype == NW_F32 && absorbing != NW_PML && g.D[2] % 4 == 0 &&
+         g.P2 % 4 == 0;
+}
+
+// 4-row tiles unless 8-row tiles already give several workgroups per CU; 8-plane chunks
+template <int TY>
+static unsigned grid3(const Geo<float>& g, int shots) {
+  const int T = div_up((int)g.D[2], 256) * div_up((int)g.D[1], TY) * div_up((int)g.D[0], 8) * shots;
+  return (unsigned)(kXcds * div_up(T, kXcds));
+}
+
+static bool rows8(const Geo<float>& g, int shots) { return grid3<8>(g, shots) >= 2048; }
+
+template <int R, int ABS>
+static void step3_t(const Geo<float>& g, const StepIO<float>& io, int shots, int xm,
+                    hipStream_t st) {
+  if (rows8(g, shots))
+    hipLaunchKernelGGL((k3d_step<R, ABS, 8, 8>), dim3(grid3<8>(g, shots)), dim3(256), 0, st, g, io,
+                       shots, xm & 1);
+  else
+    hipLaunchKernelGGL((k3d_step<R, ABS, 4, 8>), dim3(grid3<4>(g, shots)), dim3(256), 0, st, g, io,
+                       shots, xm & 1);
+}
+
+template <int R, int ABS>
+static void adj3_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int xm,
+                   hipStream_t st) {
+  if (rows8(g, shots))
+    hipLaunchKernelGGL((k3d_adjoint<R, ABS, 8, 8>), dim3(grid3<8>(g, shots)), dim3(256), 0, st, g,
+                       io, shots, xm & 1);
+  else
+    hipLaunchKernelGGL((k3d_adjoint<R, ABS, 4, 8>), dim3(grid3<4>(g, shots)), dim3(256), 0, st, g,
+                       io, shots, xm & 1);
+}
+
+void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
+                 int xm, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int key = absorbing * 2 + (order == 4 ? 1 : 0);
+  switch (key) {
+    case NW_NONE * 2 + 0: step3_t<1, NW_NONE>(g, io, shots, xm, st); break;
+    case NW_NONE * 2 + 1: step3_t<2, NW_NONE>(g, io, shots, xm, st); break;
+    case NW_SPONGE * 2 + 0: step3_t<1, NW_SPONGE>(g, io, shots, xm, st); break;
+    case NW_SPONGE * 2 + 1: step3_t<2, NW_SPONGE>(g, io, shots, xm, st); break;
+    case NW_CERJAN * 2 + 0: step3_t<1, NW_CERJAN>(g, io, shots, xm, st); break;
+    case NW_CERJAN * 2 + 1: step3_t<2, NW_CERJAN>(g, io, shots, xm, st); break;
+    default: break;  // no 3-D PML
+  }
+}
+
+void fast3d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
+                    int shots, int xm, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int key = absorbing * 2 + (order == 4 ? 1 : 0);
+  switch (key) {
+    case NW_NONE * 2 + 0: adj3_t<1, NW_NONE>(g, io, shots, xm, st); break;
+    case NW_NONE * 2 + 1: adj3_t<2, NW_NONE>(g, io, shots, xm, st); break;
+    case NW_SPONGE * 2 + 0: adj3_t<1, NW_SPONGE>(g, io, shots, xm, st); break;
+    case NW_SPONGE * 2 + 1: adj3_t<2, NW_SPONGE>(g, io, shots, xm, st); break;
+    case NW_CERJAN * 2 + 0: adj3_t<1, NW_CERJAN>(g, io, shots, xm, st); break;
+    case NW_CERJAN * 2 + 1: adj3_t<2, NW_CERJAN>(g, io, shots, xm, st); break;
+    default: break;
+  }
 }
 
 }  // namespace nw

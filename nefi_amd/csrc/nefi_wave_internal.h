@@ -173,6 +173,13 @@ struct LevelRecords {
 bool fast2d_adj_block_supported(int ty, int depth);
 void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const AdjBlockIO& io,
                           int shots, int ty, int depth, int spw, int xcd_map, void* stream);
+// fast 3-D float32 kernels (nefi_wave_2d.hip): steps without PML faces or a kept Laplacian,
+// reverse steps without an imaging term
+bool fast3d_supported(const Geo<float>& g, int absorbing, int dtype);
+void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
+                 int xcd_map, void* stream);
+void fast3d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
+                    int shots, int xcd_map, void* stream);
 void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const float* tapbuf,
                           int tap_stride, int rec_max, int shots, int levels,
                           const LevelRecords& lj, void* stream);
