@@ -1749,12 +1749,13 @@ __device__ __forceinline__ void plane_put(float* lds, const float4* v) {
 __device__ __forceinline__ int ring_slot(int z, int n) { return ((z % n) + n) % n; }
 
 // The ring march shared by both kernels: fills the ring with planes z0-R .. z0+R-1, then for
-// each plane z of the chunk puts plane z+R into the ring, starts the loads of plane z+R+1 and
-// calls cell(z, ring planes) once the ring holds z-R .. z+R.
-template <int TY, int R, typename F>
+// each plane z of the chunk puts plane z+R into the ring, starts the loads of plane z+R+1
+// (calling fetched(z+R+1) right after them) and calls cell(z, ring planes) once the ring holds
+// z-R .. z+R.
+template <int TY, int R, typename F, typename G>
 __device__ __forceinline__ void ring_march(float* ring, const float* f, const float* mul,
                                            int z0, int z1, int y0, int x0, int D0, int D1,
-                                           int D2, int P2, int64_t plane, F cell) {
+                                           int D2, int P2, int64_t plane, F cell, G fetched) {
   using S = Shape3<TY, R>;
   float4 v[S::per_thread];
   for (int z = z0 - R; z < z0 + R; ++z) {
@@ -1766,7 +1767,10 @@ __device__ __forceinline__ void ring_march(float* ring, const float* f, const fl
     __syncthreads();  // plane z-1 is computed: the slot of plane z-R-1 is free
     plane_put<TY, R>(ring + ring_slot(z + R, S::ring) * S::plane, v);
     __syncthreads();
-    if (z + 1 < z1) plane_fetch<TY, R>(f, mul, z + R + 1, y0, x0, D0, D1, D2, P2, plane, v);
+    if (z + 1 < z1) {
+      plane_fetch<TY, R>(f, mul, z + R + 1, y0, x0, D0, D1, D2, P2, plane, v);
+      fetched(z + R + 1);
+    }
     const float* pz = ring + ring_slot(z, S::ring) * S::plane;
     const float* pm1 = ring + ring_slot(z - 1, S::ring) * S::plane;
     const float* pp1 = ring + ring_slot(z + 1, S::ring) * S::plane;
@@ -1831,7 +1835,7 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
         st4(io.ck_cur + base + ro, pc);
       }
     }
-  });
+  }, [](int) {});
 }
 
 // The reverse step without an imaging term (the initial-value operator's sweep): lambda^n from
@@ -1855,7 +1859,21 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
   const float* L1 = io.lam1 + base;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
   const int tpn = io.taper_next;
-  ring_march<TY, R>(ring, L1, io.ma, z0, min(D0, z0 + ZC), y0, x0, D0, D1, D2, P2, plane,
+  const int z1 = min(D0, z0 + ZC);
+  // lambda^{n+1} at the lane's own cells of planes z .. z+R+1 (lq[0] = plane z), loaded beside
+  // the ring's loads of the same plane so that both are served by one trip to memory
+  float4 lq[R + 2][S::my_rows];
+  auto own = [&](int zz, float4* out) {
+#pragma unroll
+    for (int k = 0; k < S::my_rows; ++k) {
+      const int y = y0 + r0 + k;
+      out[k] = (act && y < D1 && zz < D0) ? ld4(L1 + ((int64_t)zz * plane + (y * P2 + x)))
+                                          : f4(0.f);
+    }
+  };
+#pragma unroll
+  for (int j = 0; j <= R; ++j) own(z0 + j, lq[j]);
+  ring_march<TY, R>(ring, L1, io.ma, z0, z1, y0, x0, D0, D1, D2, P2, plane,
                     [&](int z, const float* pz, const float* pm1, const float* pp1,
                         const float* pm2, const float* pp2) {
     const float hz = ABS != NW_NONE ? g.damp[0][z] : 0.f;
@@ -1866,7 +1884,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
       const int rr = r0 + k + R;
       const float4 lapq = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
       const int64_t ro = (int64_t)z * plane + (y * P2 + x);
-      const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
+      const float4 lc = lq[0][k], l2c = ld4(io.lam2 + base + ro);
       const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
       const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
       float4 rb;
@@ -1877,7 +1895,11 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
           adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
       st4(io.lam0 + base + ro, o);
     }
-  });
+#pragma unroll
+    for (int j = 0; j <= R; ++j)
+#pragma unroll
+      for (int k = 0; k < S::my_rows; ++k) lq[j][k] = lq[j + 1][k];
+  }, [&](int zz) { own(zz, lq[R + 1]); });
 }
 
 // ------------------------------------------------------------------------------------------
