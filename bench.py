@@ -365,17 +365,17 @@ def pat3d_bench(args, dev) -> dict:
     }
 
 
-def measured_traffic(kernel_key: str, xcd_map: int):
-    """HBM bytes per launch of one kernel instance (with the tile order ``xcd_map``) from the
-    committed rocprofv3 counter runs (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE
-    scaled by the calibration kernel)."""
+def measured_traffic(kernel_key: str, shots_per_group: int, xcd_map: int):
+    """HBM bytes per launch of one kernel instance (with ``shots_per_group`` shots per workgroup
+    and the tile order ``xcd_map``) from the committed rocprofv3 counter runs
+    (profiles/pmc_traffic_*.md, FETCH_SIZE and WRITE_SIZE scaled by the calibration kernel)."""
     for path in sorted((ROOT / "profiles").glob("pmc_traffic_*.md")):
         text = path.read_text()
         start = text.rfind("```json")
         if start < 0:
             continue
         blob = json.loads(text[start + 7 : text.index("```", start + 7)])
-        k = blob["kernels"].get(f"{kernel_key}|xcd{xcd_map}")
+        k = blob["kernels"].get(f"{kernel_key}|spw{shots_per_group}|xcd{xcd_map}")
         if k:
             return k["fetch_bytes"] + k["write_bytes"], path.name
     return None, None
@@ -500,6 +500,7 @@ def main() -> int:
     tiles["time_block"] = plan.option(nat.OPT_TIME_BLOCK)
     tiles["time_block_adjoint"] = plan.option(nat.OPT_TIME_BLOCK_ADJOINT)
     tiles["keep_laplacian"] = plan.option(nat.OPT_KEEP_LAP)
+    tiles["source"] = (plan.tune_choice or {}).get("source", "library defaults")
     dom_kind = max(("step", "steps_blocked", "adjoint", "adjoint_blocked"),
                    key=lambda k: timing[k][0])
     reverse = dom_kind.startswith("adjoint")
@@ -537,7 +538,8 @@ def main() -> int:
         inst += f", {tiles['rows_per_iter']}"
     if blocked:
         inst += f", {depth}"
-    traffic, traffic_src = measured_traffic(inst + ">", tiles["xcd_map"])
+    spw_t = tiles["adjoint" if reverse else "step"]["shots_per_group"]
+    traffic, traffic_src = measured_traffic(inst + ">", spw_t, tiles["xcd_map"])
     roofline = {
         "bound": "hbm",
         "kernel": kname,

@@ -136,6 +136,7 @@ class DevicePlan:
         lib = nat.lib()
         self.host, self.device, self.dtype = host, device, dtype
         self._hold: list[torch.Tensor] = []
+        self.tune_choice: dict | None = None  # set by the autotuner (tiles and their source)
 
         def dev(t, dt=None):
             if t is None:

@@ -9,7 +9,11 @@ uses the library's own per-launch HIP events (``nw_profile_*``), so it measures 
 alone. Results are cached in a JSON file keyed by the candidate set's version, device name,
 padded shape, shot count, order and absorbing layer
 (``$NEFI_AMD_TILE_CACHE``, default ``~/.cache/nefi_amd/tiles.json``); ``NEFI_AMD_AUTOTUNE=0``
-keeps the library's defaults.
+keeps the library's defaults. Choices tuned on an MI355X for common workloads ship with the
+package (``tiles_mi355x.json``, keyed by the architecture without its feature suffixes); a plan
+whose key is in neither table is tuned on first use. Timing ties between candidates make a
+fresh tuning differ from box to box, so the shipped table also keeps a workload's kernel
+instances, and the counter profiles measured for them, fixed.
 
 It tunes kernel launch shapes only and is unrelated to nefi's hyper-parameter package
 ``nefi/autotune``.
@@ -52,6 +56,16 @@ def _load() -> dict:
         return {}
 
 
+SHIPPED = Path(__file__).with_name("tiles_mi355x.json")
+
+
+def _shipped() -> dict:
+    try:
+        return json.loads(SHIPPED.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
 def _store(table: dict) -> None:
     path = _cache_path()
     try:
@@ -63,13 +77,18 @@ def _store(table: dict) -> None:
         pass
 
 
-def plan_key(plan) -> str:
+def plan_key(plan, base_arch: bool = False) -> str:
+    """Cache key of a plan; ``base_arch`` drops the architecture's feature suffixes
+    (``gfx950:sramecc+:xnack-`` -> ``gfx950``), as the shipped table's keys do."""
     h = plan.host
     if plan.device.type == "cuda":
         # the architecture name and CU count, which (unlike the marketing name) profilers
         # that wrap the process still report
         props = torch.cuda.get_device_properties(plan.device)
-        name = f"{getattr(props, 'gcnArchName', '') or props.name}/{props.multi_processor_count}"
+        arch = getattr(props, "gcnArchName", "") or props.name
+        if base_arch:
+            arch = arch.split(":")[0]
+        name = f"{arch}/{props.multi_processor_count}"
     else:
         name = "cpu"
     return f"v3|{name}|{h.dims[1]}x{h.dims[2]}|S{h.n_shots}|o{h.order}|a{h.absorbing}"
@@ -142,8 +161,11 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
                                  small)), "static": True}
     key = plan_key(plan)
     table = _load()
+    shipped = _shipped().get(plan_key(plan, base_arch=True))
     if key in table and not force:
-        choice = table[key]
+        choice = dict(table[key], source="cache")
+    elif shipped is not None and not force:
+        choice = dict(shipped, source="shipped")
     else:
         steps = min(steps, max(1, h.n_steps))
         best_total, choice = None, None
@@ -173,6 +195,7 @@ def tune(plan, steps: int = 3, force: bool = False) -> dict | None:
                 best_total, choice = total, pick
         table[key] = choice
         _store(table)
+        choice = dict(choice, source="tuned")
     plan.set_option(nat.OPT_XCD_MAP, int(choice.get("xcd_map", 1)))
     for kind, name in ((0, "step"), (1, "adjoint")):
         c = choice[name]

@@ -500,7 +500,7 @@ class _LeapfrogBase(Operator):
         if prop is None:
             prop = Propagator(DevicePlan(make_host(), device, dtype), self.grad_mode,
                               self.checkpoint_every)
-            autotune.tune(prop.plan)
+            prop.plan.tune_choice = autotune.tune(prop.plan)
             # each nw_forward / nw_adjoint call becomes one hipGraph launch after its first use
             if os.environ.get("NEFI_AMD_GRAPH", "1") != "0":
                 prop.plan.set_option(nat.OPT_GRAPH, 1)

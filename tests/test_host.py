@@ -228,3 +228,20 @@ def test_run_timestepping_record_and_checkpoint_gradients():
     with pytest.raises(ConfigError):
         ts.run_timestepping(step, u0, (k,), 5, 0.1, record=[7])
     assert math.isclose(ts.stable_dt_diffusion(1.0, (0.1, 0.1), safety=1.0), 0.1**2 / 4)
+
+
+def test_shipped_tile_table_holds_valid_choices():
+    from nefi_amd.physics import autotune as at
+
+    table = at._shipped()
+    assert table, "the shipped tile table is missing or unreadable"
+    for key, choice in table.items():
+        version, device, shape, shots, order, absorbing = key.split("|")
+        assert version == "v3" and ":" not in device and "/" in device
+        assert shots.startswith("S") and order in ("o2", "o4") and absorbing[0] == "a"
+        assert choice["xcd_map"] in (0, 1)
+        for kind, name in ((0, "step"), (1, "adjoint")):
+            c = choice[name]
+            cands = at.candidates(kind, int(shots[1:]), absorbing == "a3")
+            assert (c["rows"], c["shots_per_group"], c["rows_per_iter"], c["lds"],
+                    c["depth"]) in cands, (key, name)
