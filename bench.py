@@ -365,6 +365,66 @@ def pat3d_bench(args, dev) -> dict:
     }
 
 
+def datagen_bench(args, dev) -> dict:
+    """Float64 data generation on the 2x-finer grid, as nefi's wave_fwi data generator runs
+    its operator (twice the resolution, float64, no gradient): the benchmark survey's geometry
+    and model on 2048^2 cells of 5 m (+ the same 200 m sponge = 40 cells), 64 shots, forward
+    only. A bounded run of ``--nt`` samples (default 101: 200 steps at two substeps per
+    sample) gives the rate; the full 4001-sample generation is projected from it."""
+    import torch
+
+    from nefi_amd import _native as nat
+    from nefi_amd.domain import Domain
+    from nefi_amd.physics.batched import ShotBatchWaveOperator
+
+    wl = survey(1024, args.shots, 4001, absorbing=args.absorbing)
+    n_t = args.nt if args.nt != 4001 else 101
+    kw = dict(wl["kw"], n_t=n_t)
+    L = wl["L"]
+    n = 2 * wl["n"]
+    dom = Domain((n, n), ((0.0, L), (0.0, L)))
+    op = ShotBatchWaveOperator(dom, wl["shots"], grad_mode="checkpoint", **kw)
+    c = torch.tensor(np.kron(wl["c"], np.ones((2, 2))), dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        for _ in range(args.warmup):
+            op({"c": c})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y = op({"c": c})
+        torch.cuda.synchronize()
+        elapsed = (time.perf_counter() - t0) / args.steps
+        nat.profile_enable(True)
+        op({"c": c})
+        torch.cuda.synchronize()
+        ms, launches = nat.profile_read(nat.K_STEP)
+        nat.profile_enable(False)
+    assert bool(torch.isfinite(y).all())
+    padded = op.grid.padded_shape
+    updates = padded[0] * padded[1] * args.shots * op.n_steps
+    avg_s = ms / 1e3 / max(1, launches)
+    bpu = 32.0  # float64: read p^n, p^{n-1}, m; write p^{n+1}
+    achieved = bpu * padded[0] * padded[1] * args.shots / avg_s / 1e9
+    full_steps = op.n_steps * (4000 // (n_t - 1))
+    return {
+        "metric": "float64 data generation Gcell-updates/s (2x-fine 64-shot survey, forward)",
+        "value": round(updates / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(elapsed * 1e3, 2),
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2x-fine data generation: {n}^2 cells of 5 m (+40-cell sponge = "
+                               f"{padded[0]}^2), {args.shots} shots, {op.n_steps} steps "
+                               f"({n_t} samples, {op.substeps} substeps each), float64, "
+                               f"forward only"},
+        "full_generation_s": round(elapsed * full_steps / op.n_steps, 1),
+        "full_generation_steps": full_steps,
+        "roofline": {"bound": "hbm", "kernel": "k_step<double, order 4, sponge> (generic)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell_update": bpu,
+                     "avg_launch_us": round(avg_s * 1e6, 2)},
+    }
+
+
 def measured_traffic(kernel_key: str, shots_per_group: int, xcd_map: int):
     """HBM bytes per launch of one kernel instance (with ``shots_per_group`` shots per workgroup
     and the tile order ``xcd_map``) from the committed rocprofv3 counter runs
@@ -390,9 +450,10 @@ def main() -> int:
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
-    ap.add_argument("--workload", default="fwi", choices=("fwi", "pat3d"),
+    ap.add_argument("--workload", default="fwi", choices=("fwi", "pat3d", "datagen"),
                     help="fwi: the 64-shot 2-D gradient (the benchmark); pat3d: the 3-D "
-                         "photoacoustic forward + adjoint")
+                         "photoacoustic forward + adjoint; datagen: float64 forward on the "
+                         "2x-finer grid (nefi's data generator)")
     ap.add_argument("--absorbing", default="sponge", choices=("sponge", "pml", "cerjan"),
                     help="absorbing layer of the survey (the benchmark workload uses the sponge)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -428,6 +489,9 @@ def main() -> int:
         del a, b
     if args.workload == "pat3d":
         print(json.dumps(pat3d_bench(args, dev)), flush=True)
+        return 0
+    if args.workload == "datagen":
+        print(json.dumps(datagen_bench(args, dev)), flush=True)
         return 0
     wl = survey(args.n, args.shots, args.nt, absorbing=args.absorbing)
     n = wl["n"]
