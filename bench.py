@@ -400,6 +400,7 @@ def datagen_bench(args, dev) -> dict:
         ms, launches = nat.profile_read(nat.K_STEP)
         nat.profile_enable(False)
     assert bool(torch.isfinite(y).all())
+    tiled = next(iter(op._cache.values())).plan.option(nat.OPT_FAST_F64)
     padded = op.grid.padded_shape
     updates = padded[0] * padded[1] * args.shots * op.n_steps
     avg_s = ms / 1e3 / max(1, launches)
@@ -418,7 +419,9 @@ def datagen_bench(args, dev) -> dict:
                                f"forward only"},
         "full_generation_s": round(elapsed * full_steps / op.n_steps, 1),
         "full_generation_steps": full_steps,
-        "roofline": {"bound": "hbm", "kernel": "k_step<double, order 4, sponge> (generic)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k2d64_step<order 4, sponge, 16 rows> (LDS tiles)" if tiled
+                                else "k_step<double, order 4, sponge> (generic)"),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell_update": bpu,
                      "avg_launch_us": round(avg_s * 1e6, 2)},

@@ -215,6 +215,7 @@ OPT_TIME_BLOCK = 6
 OPT_KEEP_LAP = 7
 OPT_TIME_BLOCK_ADJOINT = 8
 OPT_FAST_3D = 9
+OPT_FAST_F64 = 10
 
 
 def profile_enable(on: bool) -> None:

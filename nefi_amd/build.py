@@ -15,7 +15,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 SRC = PKG / "csrc"
 OUT = PKG / "_lib" / "libnefi_wave.so"
-SOURCES = ["nefi_wave.hip", "nefi_wave_2d.hip", "nefi_filter.hip", "nefi_stepper.hip"]
+SOURCES = ["nefi_wave.hip", "nefi_wave_2d.hip", "nefi_wave_2d64.hip", "nefi_filter.hip",
+           "nefi_stepper.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

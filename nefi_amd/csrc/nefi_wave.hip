@@ -544,6 +544,7 @@ struct nw_plan {
   float* tapbuf;
   int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
   int fast3d;        // NW_OPT_FAST_3D: the 2.5-D tiled 3-D float32 kernels (when they cover it)
+  int fast64;        // NW_OPT_FAST_F64: the LDS-tiled 2-D float64 forward step
   // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth and the
   // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
   int tblock_adj;
@@ -682,6 +683,11 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
     }
     if (P->fast3d && !io.lap_out) {
       fast3d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
+      return;
+    }
+  } else {
+    if (P->fast64 && !io.lap_out) {
+      fast2d64_step(P->gd, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
       return;
     }
   }
@@ -1328,6 +1334,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tapbuf = nullptr;
   p->keep_lap_opt = -1;
   p->fast3d = fast3d_supported(p->gf, d.absorbing, d.dtype) ? 1 : 0;
+  p->fast64 = fast2d64_supported(p->gd, d.absorbing, d.dtype) ? 1 : 0;
   p->tblock_adj = 1;
   p->injv = nullptr;
   p->graph_mode = 0;
@@ -1426,6 +1433,12 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
     P->fast3d = value != 0;
     return NW_OK;
   }
+  if (option == NW_OPT_FAST_F64) {
+    if (value && !fast2d64_supported(P->gd, P->d.absorbing, P->d.dtype))
+      return fail(NW_ERR_UNSUPPORTED, "the tiled float64 step does not cover this plan");
+    P->fast64 = value != 0;
+    return NW_OK;
+  }
   if (option == NW_OPT_TIME_BLOCK || option == NW_OPT_TIME_BLOCK_ADJOINT) {
     if (value < 1 || value > kMaxDepth)
       return fail(NW_ERR_ARG, "time block depth must be 1.." + std::to_string(kMaxDepth));
@@ -1484,6 +1497,10 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_FAST_3D) {
     *value = P->fast3d;
+    return NW_OK;
+  }
+  if (option == NW_OPT_FAST_F64) {
+    *value = P->fast64;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

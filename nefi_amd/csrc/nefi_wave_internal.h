@@ -180,6 +180,10 @@ void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<flo
                  int xcd_map, void* stream);
 void fast3d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
                     int shots, int xcd_map, void* stream);
+// LDS-tiled 2-D float64 forward step (nefi_wave_2d64.hip)
+bool fast2d64_supported(const Geo<double>& g, int absorbing, int dtype);
+void fast2d64_step(const Geo<double>& g, int order, int absorbing, const StepIO<double>& io,
+                   int shots, int xcd_map, void* stream);
 void fast2d_sample_levels(const Geo<float>& g, const SampleIO<float>& io, const float* tapbuf,
                           int tap_stride, int rec_max, int shots, int levels,
                           const LevelRecords& lj, void* stream);
