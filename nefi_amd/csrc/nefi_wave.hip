@@ -673,8 +673,16 @@ static void launch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
                      io);
 }
 
+// whether the forward step of a generic-loop call gathers the receivers of its own input p^n
+// (the tiled 3-D step does, in workgroups after its tiles)
 template <typename T>
-static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
+static bool step_samples(const nw_plan* P, bool klap) {
+  return std::is_same<T, float>::value && P->rows[0] == 0 && P->fast3d && !klap;
+}
+
+template <typename T>
+static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st,
+                          const SampleIO<T>* smp = nullptr) {
   if constexpr (std::is_same<T, float>::value) {
     if (P->rows[0] > 0) {
       fast2d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->rows[0], P->spw[0],
@@ -682,7 +690,8 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st) {
       return;
     }
     if (P->fast3d && !io.lap_out) {
-      fast3d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
+      fast3d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, smp,
+                  P->d.rec_max, st);
       return;
     }
   } else {
@@ -933,8 +942,11 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
   T* prev = s0;
   T* cur = s1;
   T* psi_cur[3] = {pa[0], pa[1], pa[2]};
+  const bool fused_sample = step_samples<T>(P, klap);
   for (int n = n_begin; n < n_end; ++n) {
-    if (traces && recorded(d, n)) NW_TRY(launch_sample<T>(P, cur, traces, n / d.substeps, st));
+    const bool sample_n = traces && recorded(d, n);
+    if (sample_n && !fused_sample)
+      NW_TRY(launch_sample<T>(P, cur, traces, n / d.substeps, st));
     StepIO<T> io{};
     io.m = m;
     io.p_prev = prev;
@@ -975,7 +987,12 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
         }
     }
     const int pid = prof_begin(NW_K_STEP, st);
-    dispatch_step<T>(P, io, st);
+    if (sample_n && fused_sample) {
+      const SampleIO<T> smp = sample_io<T>(P, cur, traces, n / d.substeps);
+      dispatch_step<T>(P, io, st, &smp);
+    } else {
+      dispatch_step<T>(P, io, st);
+    }
     prof_end(pid, st);
     NW_TRY(check_launch("k_step"));
     prev = cur;

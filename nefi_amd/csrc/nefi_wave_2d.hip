@@ -1780,11 +1780,36 @@ __device__ __forceinline__ void ring_march(float* ring, const float* f, const fl
   }
 }
 
+// Receiver gather of p^n by the workgroups that follow a step launch's tiles: workgroup b of
+// them takes shot b / per_shot and the receiver taps 256 (b % per_shot) .. +255 of that shot,
+// with k_sample's arithmetic (tap products summed by the same shuffle butterfly), so a step
+// samples its own input p^n without a launch of its own.
+__device__ __forceinline__ void sample_block(const Geo<float>& g, const SampleIO<float>& io,
+                                             int b, int per_shot) {
+  const int s = b / per_shot;
+  const int t = (b % per_shot) * 256 + threadIdx.x;
+  const int taps = io.taps;
+  const int r = t / taps, k = t % taps;
+  const int count = io.rec_count[s];
+  float v = 0.f;
+  if (r < count) {
+    const int64_t row = (int64_t)io.rec_first[s] + r;
+    v = io.rec_w[row * taps + k] * io.p[(int64_t)s * g.cells + io.rec_idx[row * taps + k]];
+  }
+  for (int off = 1; off < taps; off <<= 1) v += __shfl_xor(v, off, 64);
+  if (r < count && k == 0) io.traces[((int64_t)io.trace_first[s] + r) * io.n_t + io.j] = v;
+}
+
 template <int R, int ABS, int TY, int ZC>
 __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, int shots,
-                                                int xcd_map) {
+                                                int xcd_map, SampleIO<float> smp,
+                                                int tile_blocks, int sample_per_shot) {
   using S = Shape3<TY, R>;
   __shared__ __align__(16) float ring[S::ring * S::plane];
+  if ((int)blockIdx.x >= tile_blocks) {  // uniform over the workgroup
+    sample_block(g, smp, (int)blockIdx.x - tile_blocks, sample_per_shot);
+    return;
+  }
   const int D0 = (int)g.D[0], D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int64_t plane = (int64_t)D1 * P2;
   const Tile3 tl = tile3_of(div_up(D2, 256), div_up(D1, TY), div_up(D0, ZC), shots, xcd_map);
@@ -2154,13 +2179,17 @@ static bool rows8(const Geo<float>& g, int shots) { return grid3<8>(g, shots) >=
 
 template <int R, int ABS>
 static void step3_t(const Geo<float>& g, const StepIO<float>& io, int shots, int xm,
-                    hipStream_t st) {
-  if (rows8(g, shots))
-    hipLaunchKernelGGL((k3d_step<R, ABS, 8, 8>), dim3(grid3<8>(g, shots)), dim3(256), 0, st, g, io,
-                       shots, xm & 1);
-  else
-    hipLaunchKernelGGL((k3d_step<R, ABS, 4, 8>), dim3(grid3<4>(g, shots)), dim3(256), 0, st, g, io,
-                       shots, xm & 1);
+                    const SampleIO<float>& smp, int per_shot, hipStream_t st) {
+  const int extra = per_shot * shots;  // receiver-gather workgroups after the tiles
+  if (rows8(g, shots)) {
+    const int tiles = (int)grid3<8>(g, shots);
+    hipLaunchKernelGGL((k3d_step<R, ABS, 8, 8>), dim3(tiles + extra), dim3(256), 0, st, g, io,
+                       shots, xm & 1, smp, tiles, per_shot);
+  } else {
+    const int tiles = (int)grid3<4>(g, shots);
+    hipLaunchKernelGGL((k3d_step<R, ABS, 4, 8>), dim3(tiles + extra), dim3(256), 0, st, g, io,
+                       shots, xm & 1, smp, tiles, per_shot);
+  }
 }
 
 template <int R, int ABS>
@@ -2175,16 +2204,22 @@ static void adj3_t(const Geo<float>& g, const AdjIO<float>& io, int shots, int x
 }
 
 void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
-                 int xm, void* stream) {
+                 int xm, const SampleIO<float>* sample, int rec_max, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const int key = absorbing * 2 + (order == 4 ? 1 : 0);
+  SampleIO<float> smp{};
+  int per_shot = 0;
+  if (sample && rec_max > 0) {
+    smp = *sample;
+    per_shot = div_up(rec_max * sample->taps, 256);
+  }
   switch (key) {
-    case NW_NONE * 2 + 0: step3_t<1, NW_NONE>(g, io, shots, xm, st); break;
-    case NW_NONE * 2 + 1: step3_t<2, NW_NONE>(g, io, shots, xm, st); break;
-    case NW_SPONGE * 2 + 0: step3_t<1, NW_SPONGE>(g, io, shots, xm, st); break;
-    case NW_SPONGE * 2 + 1: step3_t<2, NW_SPONGE>(g, io, shots, xm, st); break;
-    case NW_CERJAN * 2 + 0: step3_t<1, NW_CERJAN>(g, io, shots, xm, st); break;
-    case NW_CERJAN * 2 + 1: step3_t<2, NW_CERJAN>(g, io, shots, xm, st); break;
+    case NW_NONE * 2 + 0: step3_t<1, NW_NONE>(g, io, shots, xm, smp, per_shot, st); break;
+    case NW_NONE * 2 + 1: step3_t<2, NW_NONE>(g, io, shots, xm, smp, per_shot, st); break;
+    case NW_SPONGE * 2 + 0: step3_t<1, NW_SPONGE>(g, io, shots, xm, smp, per_shot, st); break;
+    case NW_SPONGE * 2 + 1: step3_t<2, NW_SPONGE>(g, io, shots, xm, smp, per_shot, st); break;
+    case NW_CERJAN * 2 + 0: step3_t<1, NW_CERJAN>(g, io, shots, xm, smp, per_shot, st); break;
+    case NW_CERJAN * 2 + 1: step3_t<2, NW_CERJAN>(g, io, shots, xm, smp, per_shot, st); break;
     default: break;  // no 3-D PML
   }
 }

@@ -176,8 +176,9 @@ void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const A
 // fast 3-D float32 kernels (nefi_wave_2d.hip): steps without PML faces or a kept Laplacian,
 // reverse steps without an imaging term
 bool fast3d_supported(const Geo<float>& g, int absorbing, int dtype);
+// fast3d_step with `sample` non-null also gathers the receivers of p^n (io.p) into the traces
 void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,
-                 int xcd_map, void* stream);
+                 int xcd_map, const SampleIO<float>* sample, int rec_max, void* stream);
 void fast3d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<float>& io,
                     int shots, int xcd_map, void* stream);
 // LDS-tiled 2-D float64 forward step (nefi_wave_2d64.hip)
