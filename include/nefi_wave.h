@@ -137,6 +137,11 @@ int32_t nw_model(nw_plan* plan, const void* c, void* m, void* ma, void* stream);
  *
  * state[0], state[1]: p^{n-1}, p^n of every shot on entry; on return the two pointers are
  *   swapped as needed so that they again hold p^{n_end-1}, p^{n_end}.
+ * PML face fields inside the layer-free box (cells whose damping, cell profile and both face
+ *   profiles vanish along every axis) are identically zero; the fast 2-D kernels neither read
+ *   nor write them there. Face buffers handed to nw_forward — psi and its spares, checkpoint
+ *   and keep slots — must hold zeros in that box (allocate them zeroed), and the psi_bar
+ *   buffers of nw_adjoint finite values.
  * psi: PML only — psi[0..ndim-1] hold the auxiliary face fields psi^{n-1/2} and
  *   psi[ndim..2ndim-1] are spare buffers of the same sizes; the pointer pairs are swapped on
  *   return like `state`. NULL otherwise.

@@ -453,15 +453,29 @@ __global__ void __launch_bounds__(256) k_clamp(int64_t cells, T* p, const int32_
 // ------------------------------------------------------------------------------------------
 // Neutral-damping interval of axes 1 and 2 of a float plan with a sponge or Cerjan layer, from
 // the device profiles: the run of neutral cells around the middle of each axis.
+// PML: a cell is neutral along axis a when its damping and cell profile vanish and so do the
+// face profiles of both its faces. In a tile of cells neutral along both axes every face-field
+// coefficient (zeta_other - zf) kappa is zero, so the face fields there stay at their initial
+// zero and enter the neighbours' updates only through zero coefficients.
 static int quiet_box(Geo<float>& g) {
+  const bool pml = g.damp_kind == NW_PML;
+  auto fetch = [](const float* src, int n, std::vector<float>& v) {
+    v.resize(n);
+    return hipMemcpy(v.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
+  };
   for (int a = 1; a < 3; ++a) {
     const int n = (int)g.D[a];
-    std::vector<float> v(n);
-    const hipError_t e = hipMemcpy(v.data(), g.damp[a], n * sizeof(float), hipMemcpyDeviceToHost);
+    std::vector<float> v, zc, zf;
+    hipError_t e = fetch(g.damp[a], n, v);
+    if (e == hipSuccess && pml) e = fetch(g.zc[a], n, zc);
+    if (e == hipSuccess && pml) e = fetch(g.zf[a], n + 1, zf);
     if (e != hipSuccess)
       return fail(NW_ERR_HIP, std::string("reading the damping profile: ") + hipGetErrorString(e));
     const bool sponge = g.damp_kind == NW_SPONGE;
-    auto neutral = [&](int i) { return sponge ? v[i] == 0.f : v[i] >= 1.f; };
+    auto neutral = [&](int i) {
+      if (pml) return v[i] == 0.f && zc[i] == 0.f && zf[i] == 0.f && zf[i + 1] == 0.f;
+      return sponge ? v[i] == 0.f : v[i] >= 1.f;
+    };
     int lo = n / 2, hi = n / 2;
     if (neutral(n / 2)) {
       while (lo > 0 && neutral(lo - 1)) --lo;

@@ -1305,6 +1305,10 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
   const PmlX px = pml_x(g, x, act);
   const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
+  // a tile in the PML's quiet box neither reads nor writes face fields: they are zero there and
+  // stay zero (face buffers, snapshots and keep slots hold zeros in the box from allocation)
+  const bool quiet = y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
+                     x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
   float4 mm[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1341,8 +1345,9 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
         const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
         const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
         const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-        const float4 oy0 = ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
-        const float4 oy1 = ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
+        const float4 oy0 = quiet ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
+        const float4 oy1 =
+            quiet ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
         float4 ny0, ny1;
 #define NW_PML_Y(c)                                                                          \
   {                                                                                          \
@@ -1352,8 +1357,10 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
   }
         NW_PML_Y(x) NW_PML_Y(y) NW_PML_Y(z) NW_PML_Y(w)
 #undef NW_PML_Y
-        st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
-        if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
+        if (!quiet) {
+          st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
+          if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
+        }
         // x faces x..x+4 of row y
         const float zcy = g.zc[1][y];
         const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
@@ -1361,20 +1368,22 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
         const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-          ox[j] = (j < 4 || x + 4 <= D2) ? ox_row[j] : 0.f;
+          ox[j] = (!quiet && (j < 4 || x + 4 <= D2)) ? ox_row[j] : 0.f;
           nx[j] = __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
         }
         lap.x = lap.x + ((nx[1] + ox[1]) - (nx[0] + ox[0])) * ih2;
         lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
         lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
         lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
-        float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
+        if (!quiet) {
+          float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
-        if (last_col) nx_row[4] = nx[4];
+          for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
+          if (last_col) nx_row[4] = nx[4];
+        }
         // the Laplacian with both face divergences: the reverse step's imaging operand
         if (io.lap_out) st4(io.lap_out + base + (y * P2 + x), lap);
-        if (io.ck_psi[1]) {
+        if (io.ck_psi[1] && !quiet) {
           st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
           if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
           float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
@@ -1443,6 +1452,11 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
   const bool kept_lap = io.lap_n != nullptr;  // the forward step's Laplacian, faces included
   const bool imaging = (io.p_n != nullptr || kept_lap) && io.grad != nullptr;
   const bool p_tile = imaging && !kept_lap;
+  // In the PML's quiet box every face coefficient is zero: the adjoint face fields there enter
+  // lambda (here and in the neighbouring tiles) only multiplied by zero, so a tile inside the
+  // box neither reads nor writes them (they keep the finite values of the initial state).
+  const bool quiet = y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
+                     x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1511,8 +1525,9 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
           const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
           const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-          const float4 i0 = ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
-          const float4 i1 = ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
+          const float4 i0 = quiet ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
+          const float4 i1 =
+              quiet ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
           float4 o0, o1;
 #define NW_PML_ADJ_Y(c)                                                                      \
   {                                                                                          \
@@ -1525,9 +1540,11 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
   }
           NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
 #undef NW_PML_ADJ_Y
-          st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
-          if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
-          if (p_tile) {
+          if (!quiet) {
+            st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
+            if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
+          }
+          if (p_tile && !quiet) {
             const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
             const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
             const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
@@ -1546,7 +1563,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           float t[5], dq[5], b[5];
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
-            const float in = (j < 4 || x + 4 <= D2) ? in_row[j] : 0.f;
+            const float in = (!quiet && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
             dq[j] = qf[j + 1] - qf[j];
             t[j] = in - ih2 * dq[j];
             b[j] = (zcy - px.zf[j]) * px.k[j];
@@ -1556,10 +1573,12 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
           lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
           float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
+          if (!quiet) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
-          if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
-          if (p_tile) {
+            for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
+            if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
+          }
+          if (p_tile && !quiet) {
             const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
             const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
             float hw[5];

@@ -36,7 +36,9 @@ struct Geo {
   const T* zf[3];
   const T* zc[3];
   // per axis, the cells [quiet[a][0], quiet[a][1]) whose damping is neutral (sponge 0, Cerjan
-  // taper >= 1, every cell without a layer): a tile inside that box steps with a = b = 1
+  // taper >= 1, every cell without a layer): a tile inside that box steps with a = b = 1.
+  // PML: the cells whose damping, cell profile and both face profiles vanish (a tile inside
+  // that box has zero face fields)
   int quiet[3][2];
   int damp_kind;  // absorbing layer (nw_absorbing)
 };

@@ -358,11 +358,17 @@ class Workspace:
             self._side = torch.cuda.Stream(device=self.plan.device)
         return self._side
 
-    def get(self, name: str, numel: int, dtype=None, zero: bool = False) -> torch.Tensor:
+    def get(self, name: str, numel: int, dtype=None, zero: bool = False,
+            zero_new: bool = False) -> torch.Tensor:
+        """Buffer ``name`` of at least ``numel`` elements; ``zero`` clears it on every call,
+        ``zero_new`` only when it is (re)allocated (checkpoint and keep slots: PML face fields
+        inside the layer-free box are never written and must read as zero)."""
         dt = dtype or self.plan.dtype
         t = self.bufs.get(name)
         if t is None or t.numel() < numel or t.dtype != dt:
-            t = torch.empty(max(1, numel), device=self.plan.device, dtype=dt)
+            self.bufs.pop(name, None)
+            alloc = torch.zeros if zero_new else torch.empty
+            t = alloc(max(1, numel), device=self.plan.device, dtype=dt)
             self.bufs[name] = t
         v = t[:numel]
         if zero:
@@ -422,7 +428,7 @@ class Propagator:
         K = grad_block(P, N, self.grad_mode, self.checkpoint_every)
         nb = -(-N // K)
         last0 = (nb - 1) * K
-        ckpt = ws.get("ckpt", (nb - 1) * P.ckpt_slot) if nb > 1 else None
+        ckpt = ws.get("ckpt", (nb - 1) * P.ckpt_slot, zero_new=True) if nb > 1 else None
         # with room for the overlapped reverse sweep's second keep buffer, this sweep keeps the
         # last two blocks, so the reverse sweep recomputes one block fewer
         two = nb > 2 and self._overlap_fits(ws, K)
@@ -431,8 +437,8 @@ class Propagator:
             P.forward(m, state, psi, 0, kept0, traces=traces, ckpt=ckpt, ckpt_every=K)
         if two:
             P.forward(m, state, psi, kept0, last0, traces=traces,
-                      keep=ws.get("keep2", (K + 1) * P.keep_slot))
-        keep = ws.get("keep", (K + 1) * P.keep_slot)
+                      keep=ws.get("keep2", (K + 1) * P.keep_slot, zero_new=True))
+        keep = ws.get("keep", (K + 1) * P.keep_slot, zero_new=True)
         P.forward(m, state, psi, last0, N, traces=traces, keep=keep)
         return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, two,
                                 _Lease(ws) if ws is self.ws else None)
@@ -501,7 +507,7 @@ class Propagator:
         buffer with block nb-2."""
         P = self.plan
         N = P.host.n_steps
-        bufs = (keep, ws.get("keep2", (K + 1) * P.keep_slot))
+        bufs = (keep, ws.get("keep2", (K + 1) * P.keep_slot, zero_new=True))
         main = torch.cuda.current_stream(P.device)
         side = ws.side_stream()
 

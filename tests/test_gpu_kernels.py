@@ -281,6 +281,39 @@ def test_fast_path_against_oracle_uneven_shots(absorbing):
     assert ey < 1e-5 and eg < 1e-4
 
 
+@pytest.mark.parametrize("order", [4, 2])
+def test_pml_quiet_tiles_match_generic_and_oracle(order):
+    """PML tiles inside the layer-free box read no face fields (the reverse tiles write none):
+    on 576 padded columns whole 256-column tiles lie inside the box, beside tiles that cross the
+    layer. Traces and gradients equal the generic kernels' (kept Laplacians and p^n tiles) and
+    the FP64 oracle's to float32 accuracy."""
+    print(f"[case PML quiet tiles o{order}]", flush=True)
+    dom, geo, c, kw = _survey("pml", order, n=(72, 560), shots=3, n_t=60)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = np.random.default_rng(6).standard_normal((rows, kw["n_t"]))
+    wt = torch.tensor(w, dtype=torch.float32, device=DEV)
+    y0, g0 = _grad(op, c, wt, tiles=[(0, 1), (0, 1)])
+    for tiles, keep_lap in (([(16, 1), (16, 1)], 1), ([(8, 2), (8, 3)], 1),
+                            ([(16, 3), (8, 1)], 0)):
+        y, g = _grad(op, c, wt, tiles=tiles, lds=(1, 1), keep_lap=keep_lap)
+        ey, eg = rel(y, y0), rel(g, g0)
+        print(f"PML o{order} tiles {tiles} kept Laplacians {keep_lap}: traces {ey:.2e}, "
+              f"gradient {eg:.2e} vs generic", flush=True)
+        assert ey < 1e-6 and eg < 1e-5
+    pk = {k: v for k, v in kw.items() if k not in ("grad_mode", "checkpoint_every")}
+    ys, gs, r0 = [], np.zeros(dom.shape), 0
+    for src, rec in geo:
+        P = Problem(dom.shape, dom.extent, rec, sources=src, **pk)
+        ys.append(P.fwi_traces(c)[0])
+        gs += P.fwi_gradient(c, w[r0 : r0 + len(rec)][None])
+        r0 += len(rec)
+    eo, ego = rel(y, np.concatenate(ys, 0)), rel(g, gs)
+    print(f"PML o{order} quiet tiles vs FP64 oracle: traces {eo:.2e}, gradient {ego:.2e}",
+          flush=True)
+    assert eo < 1e-5 and ego < 1e-4
+
+
 @pytest.mark.parametrize("absorbing", ["sponge", "pml"])
 def test_hip_graph_replays_match_eager(absorbing):
     """Every call captured once, then replayed: each replay (with eager work and syncs between
