@@ -1284,15 +1284,16 @@ __device__ __forceinline__ PmlX pml_x(const Geo<float>& g, int x, bool act) {
   return r;
 }
 
-template <int R, int TY>
-__global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> io, int shots,
-                                                   int spw, int xcd_map) {
+// One tile of k3_step_pml. Q: the tile lies in the PML's quiet box, where every face-field
+// coefficient and the damping vanish: it neither reads nor writes face fields (they are zero
+// there and stay zero; face buffers, snapshots and keep slots hold zeros in the box from
+// allocation) and steps with a = b = 1, which are the values the damped arithmetic takes there.
+template <int R, int TY, bool Q>
+__device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const StepIO<float>& io,
+                                                 int shots, int spw, const Tile& tl, float* lp) {
   using S = TileShape<TY, R>;
-  __shared__ __align__(16) float lp[S::rows * kLW];
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int F2 = D2 + 1;
-  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
-  if (!tl.valid) return;  // uniform over the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
@@ -1305,10 +1306,6 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
   const PmlX px = pml_x(g, x, act);
   const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
-  // a tile in the PML's quiet box neither reads nor writes face fields: they are zero there and
-  // stay zero (face buffers, snapshots and keep slots hold zeros in the box from allocation)
-  const bool quiet = y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
-                     x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
   float4 mm[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1341,58 +1338,60 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
         const float4 nb = lnb(lp, r0 + k + R, lane);
         float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
         const float4 pc = w[R], pm = w[R - 1], pq = w[R + 1];
-        // y faces of rows y (own) and y + 1
-        const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
-        const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
-        const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-        const float4 oy0 = quiet ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
-        const float4 oy1 =
-            quiet ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
-        float4 ny0, ny1;
+        const float zcy = Q ? 0.f : g.zc[1][y];
+        if constexpr (!Q) {
+          // y faces of rows y (own) and y + 1
+          const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
+          const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
+          const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+          const float4 oy0 = ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
+          const float4 oy1 = ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
+          float4 ny0, ny1;
 #define NW_PML_Y(c)                                                                          \
   {                                                                                          \
     ny0.c = __builtin_fmaf((zcx.c - zy0) * ky0, pc.c - pm.c, ay0 * oy0.c);                   \
     ny1.c = __builtin_fmaf((zcx.c - zy1) * ky1, pq.c - pc.c, ay1 * oy1.c);                   \
     lap.c = lap.c + ((ny1.c + oy1.c) - (ny0.c + oy0.c)) * ih1;                               \
   }
-        NW_PML_Y(x) NW_PML_Y(y) NW_PML_Y(z) NW_PML_Y(w)
+          NW_PML_Y(x) NW_PML_Y(y) NW_PML_Y(z) NW_PML_Y(w)
 #undef NW_PML_Y
-        if (!quiet) {
           st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
           if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
-        }
-        // x faces x..x+4 of row y
-        const float zcy = g.zc[1][y];
-        const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
-        float ox[5], nx[5];
-        const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
+          // x faces x..x+4 of row y
+          const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
+          float ox[5], nx[5];
+          const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-          ox[j] = (!quiet && (j < 4 || x + 4 <= D2)) ? ox_row[j] : 0.f;
-          nx[j] = __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
-        }
-        lap.x = lap.x + ((nx[1] + ox[1]) - (nx[0] + ox[0])) * ih2;
-        lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
-        lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
-        lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
-        if (!quiet) {
+          for (int j = 0; j < 5; ++j) {
+            ox[j] = (j < 4 || x + 4 <= D2) ? ox_row[j] : 0.f;
+            nx[j] =
+                __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
+          }
+          lap.x = lap.x + ((nx[1] + ox[1]) - (nx[0] + ox[0])) * ih2;
+          lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
+          lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
+          lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
           float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
 #pragma unroll
           for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
           if (last_col) nx_row[4] = nx[4];
+          if (io.ck_psi[1]) {
+            st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
+            if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
+            float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) cx[j] = ox[j];
+            if (last_col) cx[4] = ox[4];
+          }
         }
         // the Laplacian with both face divergences: the reverse step's imaging operand
         if (io.lap_out) st4(io.lap_out + base + (y * P2 + x), lap);
-        if (io.ck_psi[1] && !quiet) {
-          st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
-          if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
-          float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) cx[j] = ox[j];
-          if (last_col) cx[4] = ox[4];
+        Damp4<NW_PML> dm;
+        if constexpr (Q) {
+          dm.a = dm.b = dm.tap = f4(1.f);
+        } else {
+          dm = damp4_row<NW_PML>(g.damp[1][y], hx, dcol);
         }
-        const float hy = g.damp[1][y];
-        const Damp4<NW_PML> dm = damp4_row<NW_PML>(hy, hx, dcol);
         const int ro = y * P2 + x;
         const int tpv = io.taper_prev;
         float4 o = make_float4(
@@ -1400,11 +1399,13 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
             rhs1<NW_PML>(pc.y, pp[k].y, mm[k].y, lap.y, dm.b.y, dm.tap.y, tpv),
             rhs1<NW_PML>(pc.z, pp[k].z, mm[k].z, lap.z, dm.b.z, dm.tap.z, tpv),
             rhs1<NW_PML>(pc.w, pp[k].w, mm[k].w, lap.w, dm.b.w, dm.tap.w, tpv));
-        const float zzr = g.dt2 * zcy;
-        o.x = o.x - (zzr * zcx.x) * pc.x;
-        o.y = o.y - (zzr * zcx.y) * pc.y;
-        o.z = o.z - (zzr * zcx.z) * pc.z;
-        o.w = o.w - (zzr * zcx.w) * pc.w;
+        if constexpr (!Q) {
+          const float zzr = g.dt2 * zcy;
+          o.x = o.x - (zzr * zcx.x) * pc.x;
+          o.y = o.y - (zzr * zcx.y) * pc.y;
+          o.z = o.z - (zzr * zcx.z) * pc.z;
+          o.w = o.w - (zzr * zcx.w) * pc.w;
+        }
         for (int t = 0; t < io.src_taps; ++t) {
           const int d = io.src_idx[s * io.src_taps + t] - ro;
           if (d >= 0 && d < 4) {
@@ -1415,7 +1416,7 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
             o.w += d == 3 ? sv : 0.f;
           }
         }
-        o = mul4(dm.a, o);
+        if constexpr (!Q) o = mul4(dm.a, o);
         st4(io.p_next + base + ro, o);
         if (io.ck_prev) {
           st4(io.ck_prev + base + ro, pp[k]);
@@ -1429,15 +1430,32 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
 }
 
 template <int R, int TY>
-__global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float> io,
-                                                      int shots, int spw, int xcd_map) {
+__global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> io, int shots,
+                                                   int spw, int xcd_map) {
   using S = TileShape<TY, R>;
-  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
-  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  __shared__ __align__(16) float lp[S::rows * kLW];
+  const Tile tl =
+      tile_of(div_up((int)g.D[2], 256), div_up((int)g.D[1], TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  if (y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] && x0 >= g.quiet[2][0] &&
+      x0 + 256 <= g.quiet[2][1])
+    k3_step_pml_tile<R, TY, true>(g, io, shots, spw, tl, lp);
+  else
+    k3_step_pml_tile<R, TY, false>(g, io, shots, spw, tl, lp);
+}
+
+// One tile of k3_adjoint_pml. Q: the tile lies in the PML's quiet box, where every face
+// coefficient and the damping vanish: the adjoint face fields there enter lambda (here and in
+// the neighbouring tiles) only multiplied by zero, so the tile neither reads nor writes them
+// (they keep the finite values of the initial state) and steps with a = b = 1.
+template <int R, int TY, bool Q>
+__device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const AdjIO<float>& io,
+                                                    int shots, int spw, const Tile& tl,
+                                                    float* lq, float* lp) {
+  using S = TileShape<TY, R>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int F2 = D2 + 1;
-  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
-  if (!tl.valid) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
@@ -1452,11 +1470,6 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
   const bool kept_lap = io.lap_n != nullptr;  // the forward step's Laplacian, faces included
   const bool imaging = (io.p_n != nullptr || kept_lap) && io.grad != nullptr;
   const bool p_tile = imaging && !kept_lap;
-  // In the PML's quiet box every face coefficient is zero: the adjoint face fields there enter
-  // lambda (here and in the neighbouring tiles) only multiplied by zero, so a tile inside the
-  // box neither reads nor writes them (they keep the finite values of the initial state).
-  const bool quiet = y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
-                     x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1498,10 +1511,14 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
         const int ro = y * P2 + x;
         const float4 qn = lnb(lq, r0 + k + R, lane);
         const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
-        const float hy = g.damp[1][y];
-        const Damp4<NW_PML> dm = damp4_row<NW_PML>(hy, hx, dcol);
+        Damp4<NW_PML> dm;
+        if constexpr (Q) {
+          dm.a = dm.b = dm.tap = f4(1.f);
+        } else {
+          dm = damp4_row<NW_PML>(g.damp[1][y], hx, dcol);
+        }
         const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
-        const float zcy = g.zc[1][y];
+        const float zcy = Q ? 0.f : g.zc[1][y];
         const float zzr = g.dt2 * zcy;
         float4 rb, lam;
 #define NW_PML_ADJ(c)                                                                        \
@@ -1519,15 +1536,16 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           const float4 pn = lnb(lp, r0 + k + R, lane);
           lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
         }
+        if constexpr (!Q) {
         // y faces of rows y and y + 1
         {
           const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
           const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
           const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
           const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-          const float4 i0 = quiet ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
+          const float4 i0 = ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
           const float4 i1 =
-              quiet ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
+              ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
           float4 o0, o1;
 #define NW_PML_ADJ_Y(c)                                                                      \
   {                                                                                          \
@@ -1540,11 +1558,9 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
   }
           NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
 #undef NW_PML_ADJ_Y
-          if (!quiet) {
-            st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
-            if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
-          }
-          if (p_tile && !quiet) {
+          st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
+          if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
+          if (p_tile) {
             const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
             const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
             const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
@@ -1563,7 +1579,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           float t[5], dq[5], b[5];
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
-            const float in = (!quiet && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
+            const float in = (j < 4 || x + 4 <= D2) ? in_row[j] : 0.f;
             dq[j] = qf[j + 1] - qf[j];
             t[j] = in - ih2 * dq[j];
             b[j] = (zcy - px.zf[j]) * px.k[j];
@@ -1573,12 +1589,10 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
           lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
           lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
           float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
-          if (!quiet) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
-            if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
-          }
-          if (p_tile && !quiet) {
+          for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
+          if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
+          if (p_tile) {
             const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
             const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
             float hw[5];
@@ -1589,6 +1603,7 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
             lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
             lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
           }
+        }
         }
         st4(io.lam0 + base + ro, lam);
         if (imaging) {
@@ -1622,6 +1637,23 @@ __global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float>
       }
     }
   }
+}
+
+template <int R, int TY>
+__global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float> io,
+                                                      int shots, int spw, int xcd_map) {
+  using S = TileShape<TY, R>;
+  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
+  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  const Tile tl =
+      tile_of(div_up((int)g.D[2], 256), div_up((int)g.D[1], TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
+  if (y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] && x0 >= g.quiet[2][0] &&
+      x0 + 256 <= g.quiet[2][1])
+    k3_adjoint_pml_tile<R, TY, true>(g, io, shots, spw, tl, lq, lp);
+  else
+    k3_adjoint_pml_tile<R, TY, false>(g, io, shots, spw, tl, lq, lp);
 }
 
 // ------------------------------------------------------------------------------------------
