@@ -1306,6 +1306,9 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
   const PmlX px = pml_x(g, x, act);
   const float ih1 = g.inv2h[1], ih2 = g.inv2h[2];
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
+  // lanes whose four cells lie in the quiet box (within a tile that does not) skip their face
+  // loads and stores the same way; a row's cells are in the box when the row is
+  const bool col_quiet = x >= g.quiet[2][0] && x + 4 <= g.quiet[2][1];
   float4 mm[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1340,12 +1343,14 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
         const float4 pc = w[R], pm = w[R - 1], pq = w[R + 1];
         const float zcy = Q ? 0.f : g.zc[1][y];
         if constexpr (!Q) {
+          const bool lq = col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1];
           // y faces of rows y (own) and y + 1
           const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
           const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
           const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-          const float4 oy0 = ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
-          const float4 oy1 = ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
+          const float4 oy0 = lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
+          const float4 oy1 =
+              lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
           float4 ny0, ny1;
 #define NW_PML_Y(c)                                                                          \
   {                                                                                          \
@@ -1355,15 +1360,17 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
   }
           NW_PML_Y(x) NW_PML_Y(y) NW_PML_Y(z) NW_PML_Y(w)
 #undef NW_PML_Y
-          st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
-          if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
+          if (!lq) {
+            st4(io.psi_new[1] + fy + ((int64_t)y * D2 + x), ny0);
+            if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
+          }
           // x faces x..x+4 of row y
           const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
           float ox[5], nx[5];
           const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
-            ox[j] = (j < 4 || x + 4 <= D2) ? ox_row[j] : 0.f;
+            ox[j] = (!lq && (j < 4 || x + 4 <= D2)) ? ox_row[j] : 0.f;
             nx[j] =
                 __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
           }
@@ -1371,11 +1378,13 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
           lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
           lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
           lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
-          float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
+          if (!lq) {
+            float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
-          if (last_col) nx_row[4] = nx[4];
-          if (io.ck_psi[1]) {
+            for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
+            if (last_col) nx_row[4] = nx[4];
+          }
+          if (io.ck_psi[1] && !lq) {
             st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
             if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
             float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
@@ -1470,6 +1479,8 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   const bool kept_lap = io.lap_n != nullptr;  // the forward step's Laplacian, faces included
   const bool imaging = (io.p_n != nullptr || kept_lap) && io.grad != nullptr;
   const bool p_tile = imaging && !kept_lap;
+  // lanes whose four cells lie in the quiet box skip their face loads and stores as well
+  const bool col_quiet = x >= g.quiet[2][0] && x + 4 <= g.quiet[2][1];
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
@@ -1537,16 +1548,18 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
           lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
         }
         if constexpr (!Q) {
-        // y faces of rows y and y + 1
-        {
-          const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
-          const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
-          const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
-          const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
-          const float4 i0 = ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
-          const float4 i1 =
-              ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
-          float4 o0, o1;
+          const bool lq = col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1];
+          // y faces of rows y and y + 1
+          {
+            const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
+            const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
+            const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
+            const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+            const float4 i0 =
+                lq ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
+            const float4 i1 =
+                lq ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
+            float4 o0, o1;
 #define NW_PML_ADJ_Y(c)                                                                      \
   {                                                                                          \
     const float dq0 = qc.c - qm.c, dq1 = qp.c - qc.c;                                        \
@@ -1556,54 +1569,58 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
     o0.c = ay0 * t0 - ih1 * dq0;                                                             \
     o1.c = ay1 * t1 - ih1 * dq1;                                                             \
   }
-          NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
+            NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
 #undef NW_PML_ADJ_Y
-          st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
-          if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
-          if (p_tile) {
-            const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
-            const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
-            const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
-            const float4 w1 = ld4(io.psi_low[1] + fy + ((int64_t)(y + 1) * D2 + x));
-            lapp.x = lapp.x + ((h1.x + w1.x) - (h0.x + w0.x)) * ih1;
-            lapp.y = lapp.y + ((h1.y + w1.y) - (h0.y + w0.y)) * ih1;
-            lapp.z = lapp.z + ((h1.z + w1.z) - (h0.z + w0.z)) * ih1;
-            lapp.w = lapp.w + ((h1.w + w1.w) - (h0.w + w0.w)) * ih1;
+            if (!lq) {
+              st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
+              if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
+            }
+            if (p_tile && !lq) {
+              const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
+              const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
+              const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
+              const float4 w1 = ld4(io.psi_low[1] + fy + ((int64_t)(y + 1) * D2 + x));
+              lapp.x = lapp.x + ((h1.x + w1.x) - (h0.x + w0.x)) * ih1;
+              lapp.y = lapp.y + ((h1.y + w1.y) - (h0.y + w0.y)) * ih1;
+              lapp.z = lapp.z + ((h1.z + w1.z) - (h0.z + w0.z)) * ih1;
+              lapp.w = lapp.w + ((h1.w + w1.w) - (h0.w + w0.w)) * ih1;
+            }
           }
-        }
-        // x faces x..x+4 of row y
-        {
-          const float4 qc = qw[R];
-          const float qf[6] = {qn.y, qc.x, qc.y, qc.z, qc.w, qn.z};  // q at x-1 .. x+4
-          const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
-          float t[5], dq[5], b[5];
+          // x faces x..x+4 of row y
+          {
+            const float4 qc = qw[R];
+            const float qf[6] = {qn.y, qc.x, qc.y, qc.z, qc.w, qn.z};  // q at x-1 .. x+4
+            const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
+            float t[5], dq[5], b[5];
 #pragma unroll
-          for (int j = 0; j < 5; ++j) {
-            const float in = (j < 4 || x + 4 <= D2) ? in_row[j] : 0.f;
-            dq[j] = qf[j + 1] - qf[j];
-            t[j] = in - ih2 * dq[j];
-            b[j] = (zcy - px.zf[j]) * px.k[j];
+            for (int j = 0; j < 5; ++j) {
+              const float in = (!lq && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
+              dq[j] = qf[j + 1] - qf[j];
+              t[j] = in - ih2 * dq[j];
+              b[j] = (zcy - px.zf[j]) * px.k[j];
+            }
+            lam.x = lam.x - (b[1] * t[1] - b[0] * t[0]);
+            lam.y = lam.y - (b[2] * t[2] - b[1] * t[1]);
+            lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
+            lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
+            float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
+            if (!lq) {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
+              if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
+            }
+            if (p_tile && !lq) {
+              const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
+              const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
+              float hw[5];
+#pragma unroll
+              for (int j = 0; j < 5; ++j) hw[j] = hr[j] + wr[j];
+              lapp.x = lapp.x + (hw[1] - hw[0]) * ih2;
+              lapp.y = lapp.y + (hw[2] - hw[1]) * ih2;
+              lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
+              lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
+            }
           }
-          lam.x = lam.x - (b[1] * t[1] - b[0] * t[0]);
-          lam.y = lam.y - (b[2] * t[2] - b[1] * t[1]);
-          lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
-          lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
-          float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
-          if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
-          if (p_tile) {
-            const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
-            const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
-            float hw[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j) hw[j] = hr[j] + wr[j];
-            lapp.x = lapp.x + (hw[1] - hw[0]) * ih2;
-            lapp.y = lapp.y + (hw[2] - hw[1]) * ih2;
-            lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
-            lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
-          }
-        }
         }
         st4(io.lam0 + base + ro, lam);
         if (imaging) {
