@@ -428,6 +428,78 @@ def datagen_bench(args, dev) -> dict:
     }
 
 
+def datagen3d_bench(args, dev) -> dict:
+    """Float64 data generation for the 3-D photoacoustic workload on the 2x-finer grid
+    (photoacoustic3d's data generator: 384^3 cells of 50 um + the same 1 mm sponge = 424^3,
+    order 2, 4096 detectors, float64, forward only). A bounded run of ``--nt`` samples (default
+    51: 100 steps at two substeps per sample) gives the rate; the full 1501-sample generation is
+    projected from it."""
+    import torch
+    from scipy.ndimage import gaussian_filter, zoom
+
+    from nefi_amd import _native as nat
+    from nefi_amd.domain import Domain
+    from nefi_amd.physics.wave import WaveInitialConditionOperator
+
+    n, dx = 192, 1e-4
+    L = n * dx
+    rng = np.random.default_rng(2)
+    pert = gaussian_filter(rng.standard_normal((n, n, n)), 6.0, mode="wrap")
+    c = 1500.0 * (1.0 + 0.05 * pert / np.abs(pert).max())
+    det = [[(i + 0.5) * L / 64, (j + 0.5) * L / 64, 0.5 * dx] for i in range(64)
+           for j in range(64)]
+    c_max = float(c.max())
+    n_t = args.nt if args.nt != 4001 else 51
+    fine = 2 * n
+    op = WaveInitialConditionOperator(Domain((fine,) * 3, ((0.0, L),) * 3), det, n_t=n_t,
+                                      dt_obs=0.3 * dx / c_max, c=torch.tensor(zoom(c, 2, order=1)),
+                                      c_max=c_max, order=2, absorbing="sponge",
+                                      absorb_width=10 * dx)
+    p0 = torch.tensor(gaussian_filter(rng.standard_normal((fine,) * 3), 6.0, mode="wrap"),
+                      dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        for _ in range(args.warmup):
+            op({"p0": p0})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y = op({"p0": p0})
+        torch.cuda.synchronize()
+        elapsed = (time.perf_counter() - t0) / args.steps
+        nat.profile_enable(True)
+        op({"p0": p0})
+        torch.cuda.synchronize()
+        ms, launches = nat.profile_read(nat.K_STEP)
+        nat.profile_enable(False)
+    assert bool(torch.isfinite(y).all())
+    tiled = next(iter(op._cache.values())).plan.option(nat.OPT_FAST_F64)
+    padded = op.grid.padded_shape
+    cells = padded[0] * padded[1] * padded[2]
+    avg_s = ms / 1e3 / max(1, launches)
+    bpu = 32.0  # float64: read p^n, p^{n-1}, m; write p^{n+1}
+    achieved = bpu * cells / avg_s / 1e9
+    full_steps = op.substeps * 1500
+    return {
+        "metric": "3-D float64 data generation Gcell-updates/s (2x-fine photoacoustic, forward)",
+        "value": round(cells * op.n_steps / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(elapsed * 1e3, 2),
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2x-fine 3-D PAT data generation: {fine}^3 cells of 50 um "
+                               f"(+20-cell sponge = {padded[0]}^3), 4096 detectors, order 2, "
+                               f"{op.n_steps} steps ({n_t} samples, {op.substeps} substeps "
+                               f"each), float64, forward only"},
+        "full_generation_s": round(elapsed * full_steps / op.n_steps, 1),
+        "full_generation_steps": full_steps,
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k3d64_step<order 2, sponge, 4 rows, 8 planes> (2.5-D tiles)"
+                                if tiled else "k_step<double, order 2, sponge> (generic)"),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell_update": bpu,
+                     "avg_launch_us": round(avg_s * 1e6, 2)},
+    }
+
+
 def measured_traffic(kernel_key: str, shots_per_group: int, xcd_map: int):
     """HBM bytes per launch of one kernel instance (with ``shots_per_group`` shots per workgroup
     and the tile order ``xcd_map``) from the committed rocprofv3 counter runs
@@ -453,10 +525,12 @@ def main() -> int:
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4001)
     ap.add_argument("--ckpt", type=int, default=200)
-    ap.add_argument("--workload", default="fwi", choices=("fwi", "pat3d", "datagen"),
+    ap.add_argument("--workload", default="fwi",
+                    choices=("fwi", "pat3d", "datagen", "datagen3d"),
                     help="fwi: the 64-shot 2-D gradient (the benchmark); pat3d: the 3-D "
                          "photoacoustic forward + adjoint; datagen: float64 forward on the "
-                         "2x-finer grid (nefi's data generator)")
+                         "2x-finer grid (nefi's data generator); datagen3d: the same for the "
+                         "3-D photoacoustic workload")
     ap.add_argument("--absorbing", default="sponge", choices=("sponge", "pml", "cerjan"),
                     help="absorbing layer of the survey (the benchmark workload uses the sponge)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -495,6 +569,9 @@ def main() -> int:
         return 0
     if args.workload == "datagen":
         print(json.dumps(datagen_bench(args, dev)), flush=True)
+        return 0
+    if args.workload == "datagen3d":
+        print(json.dumps(datagen3d_bench(args, dev)), flush=True)
         return 0
     wl = survey(args.n, args.shots, args.nt, absorbing=args.absorbing)
     n = wl["n"]

@@ -208,9 +208,9 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   padded width and row pitch multiples of 4) runs the 2.5-D tiled 3-D kernels for forward
  *   steps and for reverse steps without an imaging term; 0 the generic kernels. Plans they do
  *   not cover report 0 and refuse 1.
- * NW_OPT_FAST_F64: 1 (default where it covers the plan: 2-D, float64, sponge/Cerjan/none,
- *   padded width and row pitch even) runs the LDS-tiled float64 forward step; 0 the generic
- *   one. Plans it does not cover report 0 and refuse 1. */
+ * NW_OPT_FAST_F64: 1 (default where it covers the plan: 2-D or 3-D, float64, sponge/Cerjan/
+ *   none, padded width and row pitch even) runs the LDS-tiled float64 forward step (2.5-D
+ *   tiles in 3-D); 0 the generic one. Plans it does not cover report 0 and refuse 1. */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,

@@ -86,3 +86,40 @@ def test_tiled_float64_option_is_refused_where_it_does_not_apply():
         assert plan.option(nat.OPT_FAST_F64) == 0
         with pytest.raises(Exception, match="float64"):
             plan.set_option(nat.OPT_FAST_F64, 1)
+
+
+@pytest.mark.parametrize("absorbing,order", [("sponge", 2), ("sponge", 4), ("cerjan", 2),
+                                             ("none", 4)])
+def test_tiled_float64_3d_step_matches_generic_and_oracle(absorbing, order):
+    """The 2.5-D tiled float64 step of 3-D plans (the photoacoustic data generator's path):
+    width 270 padded (three 128-column chunks, the last partly filled), 17 rows and 13 planes
+    (not multiples of the 4-row tile and the 8-plane chunk), a batch of 3 fields."""
+    from nefi_amd.physics.wave import WaveInitialConditionOperator
+
+    H = 1e-4
+    shape = (7, 11, 264) if absorbing != "none" else (13, 17, 270)
+    ext = tuple((0.0, n * H) for n in shape)
+    rng = np.random.default_rng(31)
+    det = [[rng.uniform(0.1, 0.9) * n * H for n in shape] for _ in range(11)]
+    c = 1500.0 * (1.0 + 0.05 * rng.random(shape))
+    kw = dict(n_t=30, dt_obs=0.3 * H / 1600.0, c_max=1600.0, order=order, absorbing=absorbing,
+              absorb_width=3 * H if absorbing != "none" else None)
+    op = WaveInitialConditionOperator(Domain(shape, ext), det, c=torch.tensor(c), **kw)
+    print(f"[case tiled f64 3-D {absorbing} o{order} padded {op.grid.padded_shape}]", flush=True)
+    assert op.grid.padded_shape[2] == 270
+    p0 = rng.standard_normal((3, *shape))
+    x = torch.tensor(p0, device=DEV)
+    out = {}
+    for fast in (True, False):
+        with torch.no_grad():
+            op({"p0": x})
+            plan = next(iter(op._cache.values())).plan
+            plan.set_option(nat.OPT_FAST_F64, int(fast))
+            assert plan.option(nat.OPT_FAST_F64) == int(fast)
+            out[fast] = op({"p0": x}).cpu().numpy()
+    e = rel(out[True], out[False])
+    P = Problem(shape, ext, det, **kw)
+    eo = rel(out[True], P.ic_traces(c, p0))
+    print(f"{absorbing} o{order}: tiled vs generic traces {e:.1e}, vs FP64 oracle {eo:.1e}",
+          flush=True)
+    assert e < 1e-13 and eo < 1e-11
