@@ -190,12 +190,13 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   other calls step one by one. Setting T > 1 allocates two wavefields and a small tap
  *   buffer held by the plan.
  * NW_OPT_KEEP_LAP: 1 lets keep buffers hold the Laplacian of p^n instead of p^n whenever both
- *   kernel kinds run the fast 2-D kernels (reads back as 1 only then): the forward kernels
- *   write it as they step, and the reverse kernels read it pointwise for the imaging term
- *   (same values, no stencil). A keep sweep feeding such a reverse sweep over steps [n0, n1)
+ *   kernel kinds run the fast 2-D kernels or the tiled 3-D ones (reads back as 1 only then):
+ *   the forward kernels write it as they step, and the reverse kernels read it pointwise for
+ *   the imaging term (same values, no stencil). A keep sweep feeding such a reverse sweep over steps [n0, n1)
  *   must then run forward over [n0, n1) (not n1 - 1). 0 keeps p^n: a blocked keep sweep then
  *   writes only the levels themselves, which measured faster overall. Default: 1 for PML
- *   plans (the kept Laplacian replaces p^n and both face-field levels in the reverse step),
+ *   plans (the kept Laplacian replaces p^n and both face-field levels in the reverse step) and
+ *   for plans on the tiled 3-D kernels (whose reverse step images only from kept Laplacians),
  *   0 otherwise.
  * NW_OPT_TIME_BLOCK_ADJOINT: temporal blocking depth of the fast reverse step (1 default,
  *   2..4): up to that many reverse steps per launch, with the residual injections of recorded

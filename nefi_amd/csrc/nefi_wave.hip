@@ -690,8 +690,8 @@ static void launch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
 // whether the forward step of a generic-loop call gathers the receivers of its own input p^n
 // (the tiled 3-D step does, in workgroups after its tiles)
 template <typename T>
-static bool step_samples(const nw_plan* P, bool klap) {
-  return std::is_same<T, float>::value && P->rows[0] == 0 && P->fast3d && !klap;
+static bool step_samples(const nw_plan* P) {
+  return std::is_same<T, float>::value && P->rows[0] == 0 && P->fast3d;
 }
 
 template <typename T>
@@ -703,7 +703,7 @@ static void dispatch_step(nw_plan* P, const StepIO<T>& io, hipStream_t st,
                   P->xcd_map | (P->rows_per_iter == 2 ? 2 : 0) | (P->lds_tiles[0] ? 4 : 0), st);
       return;
     }
-    if (P->fast3d && !io.lap_out) {
+    if (P->fast3d) {
       fast3d_step(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, smp,
                   P->d.rec_max, st);
       return;
@@ -735,7 +735,7 @@ static void dispatch_adj(nw_plan* P, const AdjIO<T>& io, hipStream_t st) {
                      P->xcd_map | (P->lds_tiles[1] ? 4 : 0), st);
       return;
     }
-    if (P->fast3d && !io.p_n && !io.lap_n) {  // no imaging term
+    if (P->fast3d && !io.p_n) {  // no imaging term, or one from kept Laplacians
       fast3d_adjoint(P->gf, P->d.order, P->d.absorbing, io, P->d.n_shots, P->xcd_map, st);
       return;
     }
@@ -799,8 +799,11 @@ static int hand_back(nw_plan* P, void** state, T* s0, T* s1, T* prev, T* cur, hi
 // it pointwise instead of re-applying the stencil to a p tile (PML: and instead of reading both
 // levels of both face fields, so there it is the default).
 static bool keep_lap(const nw_plan* P) {
-  const bool want = P->keep_lap_opt < 0 ? P->d.absorbing == NW_PML : P->keep_lap_opt != 0;
-  return P->d.dtype == NW_F32 && want && P->rows[0] > 0 && P->rows[1] > 0;
+  const bool tiled2d = P->rows[0] > 0 && P->rows[1] > 0;
+  const bool tiled3d = P->fast3d != 0;
+  const bool want = P->keep_lap_opt < 0 ? (P->d.absorbing == NW_PML || tiled3d)
+                                        : P->keep_lap_opt != 0;
+  return P->d.dtype == NW_F32 && want && (tiled2d || tiled3d);
 }
 
 // The blocked sweep applies when the plan asks for it and the forward tiles have a blocked
@@ -956,7 +959,7 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
   T* prev = s0;
   T* cur = s1;
   T* psi_cur[3] = {pa[0], pa[1], pa[2]};
-  const bool fused_sample = step_samples<T>(P, klap);
+  const bool fused_sample = step_samples<T>(P);
   for (int n = n_begin; n < n_end; ++n) {
     const bool sample_n = traces && recorded(d, n);
     if (sample_n && !fused_sample)

@@ -1923,6 +1923,7 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
       }
       if (ABS != NW_NONE) o = mul4(dm.a, o);
       st4(io.p_next + base + ro, o);
+      if (io.lap_out) st4(io.lap_out + base + ro, lap);
       if (io.ck_prev) {
         st4(io.ck_prev + base + ro, pp);
         st4(io.ck_cur + base + ro, pc);
@@ -1931,9 +1932,10 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
   }, [](int) {});
 }
 
-// The reverse step without an imaging term (the initial-value operator's sweep): lambda^n from
-// the stencil of q = (m a) lambda^{n+1}, which the ring holds, and the pointwise lambda^{n+1},
-// lambda^{n+2}.
+// The reverse step: lambda^n from the stencil of q = (m a) lambda^{n+1}, which the ring holds,
+// and the pointwise lambda^{n+1}, lambda^{n+2}. The imaging sum, when there is one, reads the
+// Laplacian of p^n the forward sweep kept (io.lap_n) and adds a lambda^{n+1} * lap p^n to the
+// shot's float64 gradient slice; the initial-value operator's sweep has none.
 template <int R, int ABS, int TY, int ZC>
 __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io, int shots,
                                                    int xcd_map) {
@@ -1953,6 +1955,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
   const int tpn = io.taper_next;
   const int z1 = min(D0, z0 + ZC);
+  const bool imaging = io.lap_n != nullptr && io.grad != nullptr;
   // lambda^{n+1} at the lane's own cells of planes z .. z+R+1 (lq[0] = plane z), loaded beside
   // the ring's loads of the same plane so that both are served by one trip to memory
   float4 lq[R + 2][S::my_rows];
@@ -1987,6 +1990,17 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
           adj1<ABS>(lc.z, l2c.z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
           adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
       st4(io.lam0 + base + ro, o);
+      if (imaging) {
+        const float4 lapp = ld4(io.lap_n + base + ro);
+        double2* gp = reinterpret_cast<double2*>(io.grad + base + ro);
+        double2 ga = gp[0], gb = gp[1];
+        ga.x = __builtin_fma((double)rb.x, (double)lapp.x, ga.x);
+        ga.y = __builtin_fma((double)rb.y, (double)lapp.y, ga.y);
+        gb.x = __builtin_fma((double)rb.z, (double)lapp.z, gb.x);
+        gb.y = __builtin_fma((double)rb.w, (double)lapp.w, gb.y);
+        gp[0] = ga;
+        gp[1] = gb;
+      }
     }
 #pragma unroll
     for (int j = 0; j <= R; ++j)

@@ -175,8 +175,9 @@ struct LevelRecords {
 bool fast2d_adj_block_supported(int ty, int depth);
 void fast2d_adjoint_steps(const Geo<float>& g, int order, int absorbing, const AdjBlockIO& io,
                           int shots, int ty, int depth, int spw, int xcd_map, void* stream);
-// fast 3-D float32 kernels (nefi_wave_2d.hip): steps without PML faces or a kept Laplacian,
-// reverse steps without an imaging term
+// fast 3-D float32 kernels (nefi_wave_2d.hip): steps without PML faces (writing the Laplacian
+// when io.lap_out is set), reverse steps without an imaging term or with one from kept
+// Laplacians (io.lap_n)
 bool fast3d_supported(const Geo<float>& g, int absorbing, int dtype);
 // fast3d_step with `sample` non-null also gathers the receivers of p^n (io.p) into the traces
 void fast3d_step(const Geo<float>& g, int order, int absorbing, const StepIO<float>& io, int shots,

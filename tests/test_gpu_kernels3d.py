@@ -6,7 +6,8 @@ kernels' traces and gradients, and the FP64 oracle's to float32 accuracy, for ev
 layer they cover and both orders: on grids whose width spans two 256-column chunks without
 filling the second, whose rows and planes are not multiples of the tile's, with batches small
 enough for the 4-row tiles and large enough for the 8-row tiles. The FWI operator runs the
-tiled forward and recompute sweeps beside the generic imaging reverse steps.
+whole gradient on them: tiled forward and recompute sweeps that keep Laplacians, and tiled
+reverse steps that image from them.
 """
 
 from __future__ import annotations
@@ -89,9 +90,10 @@ def test_fast_3d_initial_value_operator_against_oracle(absorbing):
 
 @pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 2)])
 def test_fast_3d_forward_in_the_fwi_gradient(absorbing, order):
-    """Tiled forward and recompute sweeps (checkpoint writes included) with the generic imaging
-    reverse steps: the same traces and speed gradient as the generic kernels throughout, and
-    the oracle's to float32 accuracy."""
+    """Tiled forward and recompute sweeps (checkpoint writes included; the keep sweep writes the
+    Laplacian of each kept level) and tiled reverse steps imaging from those kept Laplacians:
+    the same traces and speed gradient as the generic kernels (which keep p^n and image from
+    its stencil), and the oracle's to float32 accuracy."""
     shape = (12, 10, 262)
     rng = np.random.default_rng(8)
     ext = tuple((0.0, n * 10.0) for n in shape)
@@ -111,6 +113,7 @@ def test_fast_3d_forward_in_the_fwi_gradient(absorbing, order):
         op({"c": x.detach()})
         plan = next(iter(op._cache.values())).plan
         plan.set_option(nat.OPT_FAST_3D, int(fast))
+        assert plan.option(nat.OPT_KEEP_LAP) == int(fast)  # the tiled plan keeps Laplacians
         y = op({"c": x})
         (g,) = torch.autograd.grad((y * w).sum(), x)
         out[fast] = (y.detach().cpu().double().numpy(), g.cpu().double().numpy())
