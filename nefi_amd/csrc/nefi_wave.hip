@@ -82,13 +82,13 @@ int prof_begin(int kind, hipStream_t st, int64_t units) {
   if (!g_prof.on) return -1;
   ProfRec r{kind, units, g_prof.get(), g_prof.get()};
   if (!r.start || !r.stop) return -1;
-  hipEventRecord(r.start, st);
+  (void)hipEventRecord(r.start, st);  // a failure surfaces when nw_profile_read waits on it
   g_prof.recs.push_back(r);
   return (int)g_prof.recs.size() - 1;
 }
 
 void prof_end(int id, hipStream_t st) {
-  if (id >= 0) hipEventRecord(g_prof.recs[id].stop, st);
+  if (id >= 0) (void)hipEventRecord(g_prof.recs[id].stop, st);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -573,8 +573,6 @@ struct nw_plan {
 };
 
 namespace nw {
-
-static size_t esize(const nw_plan* p) { return p->d.dtype == NW_F64 ? 8 : 4; }
 
 static bool recorded(const nw_desc& d, int n) {
   return n % d.substeps == 0 && n / d.substeps < d.n_t;
@@ -1218,7 +1216,7 @@ std::vector<nw_plan*> g_plans;
 
 static void drop_graphs(nw_plan* P) {
   for (auto& e : P->graphs)
-    if (e.exec) hipGraphExecDestroy(e.exec);
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);  // cleanup: nothing to report
   P->graphs.clear();
 }
 
@@ -1243,14 +1241,14 @@ static int run_captured(nw_plan* P, std::vector<uintptr_t> key, std::vector<void
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(P->capture_stream, &graph);
     if (rc != NW_OK || ec != hipSuccess) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       if (rc != NW_OK) return rc;
       return fail(NW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
     }
     GraphEntry e;
     e.key = key;
     const hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (ei != hipSuccess)
       return fail(NW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     for (size_t a = 0; a < io_arrays.size(); ++a)
@@ -1388,7 +1386,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
 int32_t nw_plan_destroy(nw_plan* plan) {
   if (!plan) return NW_OK;
   drop_graphs(plan);
-  if (plan->capture_stream) hipStreamDestroy(plan->capture_stream);
+  if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
   for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
     if (b) (void)hipFree(b);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)

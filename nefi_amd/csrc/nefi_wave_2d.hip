@@ -35,9 +35,6 @@ __device__ __forceinline__ void st4(float* p, const float4& v) {
   *reinterpret_cast<float4*>(p) = v;
 }
 __device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
-__device__ __forceinline__ float comp(const float4& v, int k) {
-  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
-}
 
 // row y of a grid (zero outside the grid or for inactive lanes)
 // offsets within one shot's grid fit in 32 bits (nw_plan_create rejects >= 2^31 cells), which
