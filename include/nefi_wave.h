@@ -19,6 +19,8 @@
  *    grids (extent + 1 along their own axis), also `n_shots` consecutive; a face buffer is
  *    rounded up to a multiple of 64 elements, so every slot offset stays 256-byte aligned.
  *  - `dtype` selects float (0) or double (1) for every real-valued buffer of a plan.
+ *  - Device buffers (wavefields, face fields, model, traces, slots) must be 16-byte aligned:
+ *    the tiled kernels move 16-byte vectors. Allocations from hipMalloc or PyTorch are.
  *  - Every call is stream-ordered on `stream` (a hipStream_t passed as void*; NULL = default
  *    stream) and does not synchronise the host, except `nw_plan_create`.
  *  - Every call returns 0 on success or a nonzero code; `nw_last_error()` returns the message
