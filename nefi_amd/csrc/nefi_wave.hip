@@ -1265,6 +1265,15 @@ static int run_captured(nw_plan* P, std::vector<uintptr_t> key, std::vector<void
 
 }  // namespace nw
 
+// the tiled kernels move 16-byte vectors: every device buffer they touch is 16-byte aligned
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+static bool all_aligned16(void* const* bufs, int n) {
+  for (int i = 0; i < n; ++i)
+    if (bufs[i] && !aligned16(bufs[i])) return false;
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
@@ -1582,8 +1591,11 @@ int32_t nw_forward(nw_plan* P, const void* m, void** state, void** psi, int32_t 
   if (P->d.absorbing == NW_PML && !psi) return fail(NW_ERR_ARG, "nw_forward: PML needs psi buffers");
   if (keep && keep_begin != n_begin) return fail(NW_ERR_ARG, "nw_forward: keep_begin must equal n_begin");
   if (ckpt && ckpt_every < 1) return fail(NW_ERR_ARG, "nw_forward: ckpt_every must be >= 1");
-  hipStream_t st = (hipStream_t)stream;
   const int npsi = P->d.absorbing == NW_PML ? 2 * P->d.ndim : 0;
+  if (!aligned16(m) || !all_aligned16(state, 2) || (npsi && !all_aligned16(psi, npsi)) ||
+      !aligned16(ckpt) || !aligned16(keep))
+    return fail(NW_ERR_ARG, "nw_forward: device buffers must be 16-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
   std::vector<uintptr_t> key = {0, (uintptr_t)m, (uintptr_t)n_begin, (uintptr_t)n_end,
                                 (uintptr_t)traces, (uintptr_t)ckpt, (uintptr_t)ckpt_every,
                                 (uintptr_t)keep, (uintptr_t)keep_begin, (uintptr_t)first_step};
@@ -1608,8 +1620,11 @@ int32_t nw_adjoint(nw_plan* P, const void* m, const void* ma, void** lam, void**
   if (P->d.absorbing == NW_PML && !psi_bar) return fail(NW_ERR_ARG, "nw_adjoint: PML needs psi_bar");
   if (keep && keep_begin > n_begin) return fail(NW_ERR_ARG, "nw_adjoint: keep must start <= n_begin");
   if (grad_m && !keep) return fail(NW_ERR_ARG, "nw_adjoint: grad_m needs the kept wavefields");
-  hipStream_t st = (hipStream_t)stream;
   const int npsi = P->d.absorbing == NW_PML ? 2 * P->d.ndim : 0;
+  if (!aligned16(ma) || !all_aligned16(lam, 2) || (npsi && !all_aligned16(psi_bar, npsi)) ||
+      !aligned16(keep) || !aligned16(grad_m))
+    return fail(NW_ERR_ARG, "nw_adjoint: device buffers must be 16-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
   std::vector<uintptr_t> key = {1, (uintptr_t)ma, (uintptr_t)n_begin, (uintptr_t)n_end,
                                 (uintptr_t)keep, (uintptr_t)keep_begin, (uintptr_t)residual,
                                 (uintptr_t)inject_end, (uintptr_t)grad_m, (uintptr_t)first_step};

@@ -387,3 +387,25 @@ def test_overlapped_reverse_sweep_is_bit_identical(absorbing, monkeypatch):
     assert "keep2" in prop.ws.bufs and prop.plan.option(nat.OPT_GRAPH) in (0, 1)
     for g in grads["1"]:
         assert torch.equal(g, grads["0"][0])
+
+
+def test_misaligned_buffers_are_refused():
+    """The tiled kernels move 16-byte vectors: nw_forward / nw_adjoint refuse a buffer that is
+    not 16-byte aligned with an error instead of launching."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey("sponge", 4, n=(40, 88), shots=2, n_t=10)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    op({"c": torch.tensor(c, dtype=torch.float32, device=DEV)})
+    plan = next(iter(op._cache.values())).plan
+    m = plan.zeros(plan.host.cells + 4)
+    state, _ = plan.new_state()
+    sp = nat.ptr_array([state[0].data_ptr(), state[1].data_ptr()])
+    rc = nat.lib().nw_forward(plan.handle, m.data_ptr() + 4, sp, None, 0, 1, None, None, 0, None,
+                              0, 0, plan.stream())
+    assert rc != 0 and b"16-byte" in nat.lib().nw_last_error()
+    lp = nat.ptr_array([state[0].data_ptr() + 8, state[1].data_ptr()])
+    rc = nat.lib().nw_adjoint(plan.handle, m.data_ptr(), m.data_ptr(), lp, None, 0, 1, None, 0,
+                              None, 0, None, 0, plan.stream())
+    assert rc != 0 and b"16-byte" in nat.lib().nw_last_error()
+    plan.forward(m[: plan.host.cells], state, [], 0, 1)  # aligned buffers still run
