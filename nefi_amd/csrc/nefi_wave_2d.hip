@@ -713,6 +713,54 @@ struct BlockShape {
   static constexpr int per_thread = (n4 + threads - 1) / threads;
 };
 
+// Tile columns of k4_steps: `full` columns of out_w output columns each and, when the grid's
+// last columns fit in half a wavefront row (128 - 2 hx of them), one split column. A split
+// workgroup steps two half-width tiles of that column, row tiles 2j (lanes 0-31) and 2j + 1
+// (lanes 32-63), so the narrow rest of the grid takes half as many workgroups.
+struct BlockCols {
+  int full;
+  bool split;
+};
+template <class S>
+__host__ __device__ inline BlockCols block_cols(int D2) {
+  const int full = D2 / S::out_w, rest = D2 - full * S::out_w;
+  if (rest == 0) return {full, false};
+  if (rest <= 128 - 2 * S::hx) return {full, true};
+  return {full + 1, false};
+}
+__host__ __device__ inline int block_tiles(BlockCols c, int nY, int nG) {
+  return (c.split ? (nY + 1) / 2 * (2 * c.full + 1) : c.full * nY) * nG;
+}
+// Tile of this workgroup for block_cols' layout, per shot group; the XCD map as in tile_of.
+// With a split column, each pair of row tiles lists its full tiles and then its split tile,
+// so the split workgroups are spread over the launch rather than left to its end (for an odd
+// number of row tiles, the last pair's second row of full tiles is listed and returns).
+__device__ __forceinline__ Tile tile_of_cols(BlockCols c, int nY, int nG, int xcd_map) {
+  const int per_grp = block_tiles(c, nY, 1);
+  const int T = per_grp * nG;
+  const int per = (T + kXcds - 1) / kXcds;
+  const int i = blockIdx.x;
+  const int t = xcd_map ? (i % kXcds) * per + i / kXcds : i;
+  Tile r;
+  r.grp = t / per_grp;
+  const int u = t - r.grp * per_grp;
+  if (!c.split) {
+    r.cx = u % c.full;
+    r.ry = u / c.full;
+  } else {
+    const int pair = u / (2 * c.full + 1), v = u - pair * (2 * c.full + 1);
+    if (v < 2 * c.full) {
+      r.cx = v % c.full;
+      r.ry = 2 * pair + v / c.full;
+    } else {
+      r.cx = c.full;
+      r.ry = 2 * pair;
+    }
+  }
+  r.valid = t < T && r.ry < nY;
+  return r;
+}
+
 typedef __attribute__((address_space(1))) void gmem_void;
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -746,18 +794,27 @@ __device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, i
 // damped arithmetic at those coefficients (products by 1 are exact). Quiet tiles also
 // double-buffer the region: the next shot's p^n streams into the second LDS buffer and its
 // p^{n-1} rows into registers while the current shot computes.
-template <int R, int ABS0, int TY, int T, bool Q>
+// SP ("split", never quiet): the workgroup steps two half-width tiles of block_cols' split
+// column. LDS row r holds region row r of the upper tile in float4 columns 0-32 (lanes 0-31
+// and the left margin) and of the lower tile in columns 33-65 (lanes 32-63 and the right
+// margin). The x-neighbour reads of lanes 31 and 32 then see the other tile's edge column;
+// like the margins, that only reaches region columns within R T of the region's edge, which
+// the output columns are not (hx >= R T).
+template <int R, int ABS0, int TY, int T, bool Q, bool SP = false>
 __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, int shots,
                                         int spw, const Tile& tl, float* lpa, float* lpb) {
+  static_assert(!(Q && SP), "split tiles lie on the grid's edge");
   constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = BlockShape<TY, R, T>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ry0 = tl.ry * TY - S::hy;        // grid row of region row 0
+  const int hl = SP ? (lane & 31) : lane;    // lane within its tile
+  const int ryA = tl.ry * TY - S::hy;        // grid row of region row 0 (upper tile if SP)
+  const int ry0 = SP && lane >= 32 ? ryA + TY : ryA;  // ... of this lane's tile
   const int rx0 = tl.cx * S::out_w - S::hx;  // grid column of region column 0
-  const int x = rx0 + 4 * lane;
+  const int x = rx0 + 4 * hl;
   const bool inx = Q || (x >= 0 && x < D2);
-  const bool outx = inx && lane >= S::hx / 4 && lane < 64 - S::hx / 4;
+  const bool outx = inx && hl >= S::hx / 4 && hl < (SP ? 32 : 64) - S::hx / 4;
   const int c0 = wave * S::mr;  // first region row of this wavefront
   const float4 hx = (ABS != NW_NONE && inx) ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<ABS> dcol = damp4_col<ABS>(hx);
@@ -809,7 +866,9 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
       for (int k = 0; k < S::per_thread; ++k) {
         const int i = threadIdx.x + S::threads * k;
         const int row = i / (kLW / 4), c4 = i % (kLW / 4);
-        const int gy = ry0 + row, gx = rx0 - 4 + 4 * c4;
+        const bool lower = SP && c4 >= kLW / 8;  // the lower tile's half of the LDS row
+        const int gy = (lower ? ryA + TY : ryA) + row;
+        const int gx = lower ? rx0 + 4 * (c4 - kLW / 8) : rx0 - 4 + 4 * c4;
         const bool in = (S::threads * (k + 1) <= S::n4 || i < S::n4);
         const bool ok = in && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
         v[k] = ok ? ld4(P + (gy * P2 + gx)) : f4(0.f);
@@ -841,9 +900,18 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
       if (e < n_tap) {
         const int idx = io.rec_idx[tap0 + e];
         const int ty = idx / P2, tx = idx - ty * P2;
-        if (ty >= ry0 + S::hy && ty < ry0 + S::hy + TY && tx >= rx0 + S::hx &&
-            tx < rx0 + 256 - S::hx)
+        if constexpr (SP) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int ryh = ryA + h * TY;
+            if (ty >= ryh + S::hy && ty < ryh + S::hy + TY && tx >= rx0 + S::hx &&
+                tx < rx0 + 128 - S::hx)
+              toff[c] = (R + ty - ryh) * kLW + 4 + 128 * h + (tx - rx0);
+          }
+        } else if (ty >= ry0 + S::hy && ty < ry0 + S::hy + TY && tx >= rx0 + S::hx &&
+                   tx < rx0 + 256 - S::hx) {
           toff[c] = (R + ty - ry0) * kLW + 4 + (tx - rx0);
+        }
       }
     }
     if constexpr (!Q) __syncthreads();
@@ -890,7 +958,17 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           const int c = c0 + k;
           if (lap_out && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
             st4(lap_out + base + (y * P2 + x), lap);
-          const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+          float hy = 0.f;
+          if constexpr (ABS != NW_NONE) {
+            if constexpr (SP) {  // both tiles' rows by scalar loads (clamped), picked per lane
+              const int ya = min(max(ryA + c0 + k, 0), D1 - 1);
+              const int yb = min(max(ryA + TY + c0 + k, 0), D1 - 1);
+              const float da = g.damp[1][ya], db = g.damp[1][yb];
+              hy = lane >= 32 ? db : da;
+            } else {
+              hy = g.damp[1][y];
+            }
+          }
           const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
           const float4 pc = cur[k];
           o = make_float4(rhs1<ABS>(pc.x, pv[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
@@ -960,7 +1038,8 @@ __global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g
   __shared__ __align__(16) float lpa[S::lrows * kLW];
   __shared__ __align__(16) float lpb[S::lrows * kLW];
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
-  const Tile tl = tile_of(div_up(D2, S::out_w), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  const BlockCols cols = block_cols<S>(D2);
+  const Tile tl = tile_of_cols(cols, div_up(D1, TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
   for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
     lpa[i] = 0.f;
@@ -972,7 +1051,9 @@ __global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g
   const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
                      ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&
                      rx0 >= g.quiet[2][0] && rx0 + 256 <= g.quiet[2][1];
-  if (quiet)
+  if (cols.split && tl.cx == cols.full)
+    k4_tile<R, ABS, TY, T, false, true>(g, io, shots, spw, tl, lpa, lpb);
+  else if (quiet)
     k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lpa, lpb);
   else
     k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lpa, lpb);
@@ -2143,7 +2224,7 @@ static void launch_steps_t(const Geo<float>& g, const BlockIO& io, int shots, in
                            hipStream_t st) {
   using S = BlockShape<TY, R, T>;
   const int tiles =
-      div_up((int)g.D[2], S::out_w) * div_up((int)g.D[1], TY) * div_up(shots, spw);
+      block_tiles(block_cols<S>((int)g.D[2]), div_up((int)g.D[1], TY), div_up(shots, spw));
   hipLaunchKernelGGL((k4_steps<R, ABS, TY, T>), dim3((unsigned)(kXcds * div_up(tiles, kXcds))),
                      dim3(S::threads), 0, st, g, io, shots, spw, xm & 1);
 }

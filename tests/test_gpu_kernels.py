@@ -142,6 +142,53 @@ def test_time_blocking_with_2048_receiver_taps_per_shot():
     assert nat.OPT_TIME_BLOCK == 6
 
 
+@pytest.mark.parametrize("absorbing,order,n", [("sponge", 4, (70, 328)), ("cerjan", 4, (90, 84)),
+                                               ("none", 2, (54, 612)), ("sponge", 2, (38, 200))])
+def test_time_blocking_split_column_is_bit_identical(absorbing, order, n):
+    """Grids whose last columns fit in half a wavefront row: blocked launches step them as
+    split workgroups (two half-width tiles, row tiles 2j and 2j + 1), with an odd number of
+    row tiles for some heights, so the last split workgroup's second tile lies below the grid.
+    Sources and receivers fall in the split column too. Traces and gradient equal those of one
+    launch per step exactly. Padded widths: 344 and 100 (order 4: one or no full column, then
+    96-104 split columns), 612 (order 2: two full columns of 248, then 116 split) and 216
+    (order 2: 216 columns do not fit in half a row, so one full column, as before)."""
+    from nefi_amd import _native as nat
+
+    rng = np.random.default_rng(21)
+    h = 10.0
+    ext = ((0.0, n[0] * h), (0.0, n[1] * h))
+    geo = []
+    for s in range(4):
+        # source columns over the whole width, the last shots' in the rightmost 40 cells
+        col = rng.uniform(0.05, 0.95) if s < 2 else 1.0 - rng.uniform(5.0, 40.0) / n[1]
+        src = [[rng.uniform(0.1, 0.9) * ext[0][1], col * ext[1][1]]]
+        r = int(rng.integers(5, 60))
+        rec = [[rng.uniform(h, ext[0][1] - h), rng.uniform(h, ext[1][1] - h)] for _ in range(r)]
+        geo.append((src, rec))
+    c = 1500.0 + 300.0 * rng.random(n)
+    kw = dict(n_t=60, dt_obs=0.002, f0=15.0, c_max=1800.0, order=order, absorbing=absorbing,
+              absorb_width=80.0 if absorbing != "none" else None, grad_mode="checkpoint",
+              checkpoint_every=13)
+    print(f"[case split-column {absorbing} o{order} {n}]", flush=True)
+    op = ShotBatchWaveOperator(Domain(n, ext), geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(22))
+    for ty, spw, depth, xcd in ((32, 4, 4, 1), (16, 1, 4, 0), (8, 3, 3, 1), (32, 2, 2, 1),
+                                (16, 4, 2, 0)):
+        tiles = [(ty, spw), (16, 2)]
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=(1, 1), xcd=xcd, depth=1)
+        nat.profile_enable(True)
+        y, g = _grad(op, c, w, tiles=tiles, lds=(1, 1), xcd=xcd, depth=depth)
+        blocked = nat.profile_units(nat.K_STEPS)
+        nat.profile_enable(False)
+        print(f"split {absorbing} o{order} {n} rows {ty} spw {spw} depth {depth} xcd {xcd}: "
+              f"{blocked} blocked steps, traces {rel(y, y0):.1e}, gradient {rel(g, g0):.1e}",
+              flush=True)
+        assert blocked > 0
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
+
+
 @pytest.mark.parametrize("absorbing,order", [("sponge", 4), ("cerjan", 4), ("none", 2)])
 def test_kept_laplacian_matches_kept_wavefield(absorbing, order):
     """Keep buffers holding the Laplacian of p^n (written by the forward kernels, read pointwise
