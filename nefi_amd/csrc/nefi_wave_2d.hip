@@ -118,6 +118,62 @@ __device__ __forceinline__ Tile tile_of(int nX, int nY, int nG, int xcd_map) {
 
 __host__ __device__ inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// Tile columns of k4_steps: `full` columns of out_w output columns each and, when the grid's
+// last columns fit in half a wavefront row (128 - 2 hx of them), one split column. A split
+// workgroup steps two half-width tiles of that column, row tiles 2j (lanes 0-31) and 2j + 1
+// (lanes 32-63), so the narrow rest of the grid takes half as many workgroups.
+struct BlockCols {
+  int full;
+  bool split;
+};
+template <class S>
+__host__ __device__ inline BlockCols block_cols(int D2) {
+  const int full = D2 / S::out_w, rest = D2 - full * S::out_w;
+  if (rest == 0) return {full, false};
+  if (rest <= 128 - 2 * S::hx) return {full, true};
+  return {full + 1, false};
+}
+__host__ __device__ inline int block_tiles(BlockCols c, int nY, int nG) {
+  return (c.split ? (nY + 1) / 2 * (2 * c.full + 1) : c.full * nY) * nG;
+}
+// Tile of this workgroup for block_cols' layout, per shot group; the XCD map as in tile_of.
+// With a split column, each pair of row tiles lists its full tiles and then its split tile,
+// so the split workgroups are spread over the launch rather than left to its end (for an odd
+// number of row tiles, the last pair's second row of full tiles is listed and returns).
+__device__ __forceinline__ Tile tile_of_cols(BlockCols c, int nY, int nG, int xcd_map) {
+  const int per_grp = block_tiles(c, nY, 1);
+  const int T = per_grp * nG;
+  const int per = (T + kXcds - 1) / kXcds;
+  const int i = blockIdx.x;
+  const int t = xcd_map ? (i % kXcds) * per + i / kXcds : i;
+  Tile r;
+  r.grp = t / per_grp;
+  const int u = t - r.grp * per_grp;
+  if (!c.split) {
+    r.cx = u % c.full;
+    r.ry = u / c.full;
+  } else {
+    const int pair = u / (2 * c.full + 1), v = u - pair * (2 * c.full + 1);
+    if (v < 2 * c.full) {
+      r.cx = v % c.full;
+      r.ry = 2 * pair + v / c.full;
+    } else {
+      r.cx = c.full;
+      r.ry = 2 * pair;
+    }
+  }
+  r.valid = t < T && r.ry < nY;
+  return r;
+}
+// Tile columns of k3_adjoint (256 output columns each): the last 128 or fewer columns of the
+// grid are stepped as split workgroups, two half-width tiles each.
+__host__ __device__ inline BlockCols adj_cols(int D2) {
+  const int full = D2 / 256, rest = D2 - full * 256;
+  if (rest == 0) return {full, false};
+  if (rest <= 128) return {full, true};
+  return {full + 1, false};
+}
+
 __device__ __forceinline__ float2 ld2(const float* p) {
   return *reinterpret_cast<const float2*>(p);
 }
@@ -392,12 +448,17 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
 // ------------------------------------------------------------------------------------------
 constexpr int kLW = 256 + 8;  // LDS row: 4 margin columns on each side, 16-byte aligned
 
+// LDS row of a split tile (two half-width tiles side by side, each with its 4 margin columns)
+constexpr int kLW2 = 2 * (128 + 8);
+
 template <int TY, int R>
 struct TileShape {
   static constexpr int rows = TY + 2 * R;
   static constexpr int n4 = rows * (kLW / 4);  // float4 elements of one tile
   static constexpr int per_thread = (n4 + 255) / 256;
   static constexpr int my_rows = TY / 4;
+  static constexpr int n4s = rows * (kLW2 / 4);  // ... of a split tile
+  static constexpr int per_thread_s = (n4s + 255) / 256;
 };
 
 // issue every float4 load of a tile (rows y0-R .. y0+TY+R-1, columns x0-4 .. x0+259)
@@ -523,16 +584,36 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
 // One tile of k3_adjoint. Q tiles (tile, halo and margins inside the grid, output cells in the
 // box of neutral damping) skip every bounds check and step with a = b = 1, as k4_tile does.
 // The tile of m a_eff is shot-independent: it stays in registers for the whole shot group.
-template <int R, int ABS0, int TY, bool Q>
+// SP ("split", never quiet): two half-width tiles of the grid's last 128 or fewer columns, row
+// tiles 2j (lanes 0-31) and 2j + 1 (lanes 32-63). An LDS row (kLW2 floats) holds both tiles'
+// rows, each between its own margin columns, so every lane's x-neighbours are its own tile's.
+template <int R, int ABS0, int TY, bool Q, bool SP = false>
 __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<float>& io,
                                             int shots, int spw, const Tile& tl, float* lq,
                                             float* lp) {
+  static_assert(!(Q && SP), "split tiles lie on the grid's edge");
   constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = TileShape<TY, R>;
+  constexpr int LW = SP ? kLW2 : kLW;                             // LDS row pitch
+  constexpr int PT = SP ? S::per_thread_s : S::per_thread;        // tile loads per thread
+  constexpr int N4 = SP ? S::n4s : S::n4;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x0 = tl.cx * 256, y0 = tl.ry * TY;
-  const int x = x0 + 4 * lane;
+  const int hl = SP ? (lane & 31) : lane;  // lane within its tile
+  const int x0 = tl.cx * 256, yA = tl.ry * TY;
+  const int y0 = SP && lane >= 32 ? yA + TY : yA;
+  const int x = x0 + 4 * hl;
+  // this lane's float offset in an LDS row: 4 margin floats, then its 4 cells (the lower tile
+  // of a split row starts after the upper tile's right margin and its own left margin)
+  const int lo = 4 * lane + 4 + (SP && lane >= 32 ? 8 : 0);
+  auto row4 = [&](const float* lds, int r) {
+    return *reinterpret_cast<const float4*>(lds + r * LW + lo);
+  };
+  auto nb4 = [&](const float* lds, int r) {
+    const float2 a = *reinterpret_cast<const float2*>(lds + r * LW + lo - 2);
+    const float2 b = *reinterpret_cast<const float2*>(lds + r * LW + lo + 4);
+    return make_float4(a.x, a.y, b.x, b.y);
+  };
   const int r0 = wave * S::my_rows;
   const bool act = Q || x < D2;
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
@@ -544,16 +625,28 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   // tile loads: every element inside the grid for Q tiles
   auto load_tile = [&](const float* src, float4* v) {
 #pragma unroll
-    for (int k = 0; k < S::per_thread; ++k) {
+    for (int k = 0; k < PT; ++k) {
       const int i = threadIdx.x + 256 * k;
-      const int row = i / (kLW / 4), c4 = i % (kLW / 4);
-      const int gy = y0 - R + row, gx = x0 - 4 + 4 * c4;
-      const bool in = 256 * (k + 1) <= S::n4 || i < S::n4;
+      const int row = i / (LW / 4);
+      int c4 = i % (LW / 4), ty0 = yA;
+      if (SP && c4 >= kLW2 / 8) {  // the lower tile's half of the LDS row
+        c4 -= kLW2 / 8;
+        ty0 = yA + TY;
+      }
+      const int gy = ty0 - R + row, gx = x0 - 4 + 4 * c4;
+      const bool in = 256 * (k + 1) <= N4 || i < N4;
       const bool ok = in && (Q || (gy >= 0 && gy < D1 && gx >= 0 && gx < D2));
       v[k] = ok ? ld4(src + (gy * P2 + gx)) : f4(0.f);
     }
   };
-  float4 vm[S::per_thread];
+  auto store_tile = [&](float* lds, const float4* v) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < N4) *reinterpret_cast<float4*>(lds + 4 * i) = v[k];
+    }
+  };
+  float4 vm[PT];
   load_tile(io.ma, vm);
   // float64 imaging sums of this wavefront's cells over the shot group
   double2 ga[S::my_rows], gb[S::my_rows];
@@ -567,17 +660,17 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
     const int64_t base = (int64_t)s * g.cells;
     const float* L1 = io.lam1 + base;
     {
-      float4 vl[S::per_thread];
+      float4 vl[PT];
       load_tile(L1, vl);
       __syncthreads();  // the previous shot's reads of lq / lp are done
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
-      tile_store<TY, R>(lq, vl);
+      for (int k = 0; k < PT; ++k) vl[k] = mul4(vm[k], vl[k]);
+      store_tile(lq, vl);
     }
     if (p_tile) {
-      float4 vp[S::per_thread];
+      float4 vp[PT];
       load_tile(io.p_n + base, vp);
-      tile_store<TY, R>(lp, vp);
+      store_tile(lp, vp);
     }
     float4 lc[S::my_rows], l2c[S::my_rows], lpn[S::my_rows];
 #pragma unroll
@@ -592,16 +685,16 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
     float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
-      qw[k] = lrow(lq, r0 + k, lane);
-      pw[k] = p_tile ? lrow(lp, r0 + k, lane) : f4(0.f);
+      qw[k] = row4(lq, r0 + k);
+      pw[k] = p_tile ? row4(lp, r0 + k) : f4(0.f);
     }
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
-      qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
-      pw[2 * R] = p_tile ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
+      qw[2 * R] = row4(lq, r0 + k + 2 * R);
+      pw[2 * R] = p_tile ? row4(lp, r0 + k + 2 * R) : f4(0.f);
       if (Q || (act && y < D1)) {
-        const float4 qn = lnb(lq, r0 + k + R, lane);
+        const float4 qn = nb4(lq, r0 + k + R);
         const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
         const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
         const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
@@ -615,7 +708,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
         if (imaging) {
           float4 lapp = lpn[k];
           if (p_tile) {
-            const float4 pn = lnb(lp, r0 + k + R, lane);
+            const float4 pn = nb4(lp, r0 + k + R);
             lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
           }
           ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
@@ -654,16 +747,19 @@ template <int R, int ABS, int TY>
 __global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io, int shots,
                                                   int spw, int xcd_map) {
   using S = TileShape<TY, R>;
-  __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
-  __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
+  __shared__ __align__(16) float lq[S::rows * kLW2];  // q = (m a_eff) lambda^{n+1}
+  __shared__ __align__(16) float lp[S::rows * kLW2];  // p^n
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
-  const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
+  const BlockCols cols = adj_cols(D2);
+  const Tile tl = tile_of_cols(cols, div_up(D1, TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const bool quiet = y0 - R >= 0 && y0 + TY + R <= D1 && x0 >= 4 && x0 + 260 <= D2 &&
                      y0 >= g.quiet[1][0] && y0 + TY <= g.quiet[1][1] &&
                      x0 >= g.quiet[2][0] && x0 + 256 <= g.quiet[2][1];
-  if (quiet)
+  if (cols.split && tl.cx == cols.full)
+    k3_adj_tile<R, ABS, TY, false, true>(g, io, shots, spw, tl, lq, lp);
+  else if (quiet)
     k3_adj_tile<R, ABS, TY, true>(g, io, shots, spw, tl, lq, lp);
   else
     k3_adj_tile<R, ABS, TY, false>(g, io, shots, spw, tl, lq, lp);
@@ -712,54 +808,6 @@ struct BlockShape {
   static constexpr int n4 = crow * (kLW / 4);  // float4 loads of one region
   static constexpr int per_thread = (n4 + threads - 1) / threads;
 };
-
-// Tile columns of k4_steps: `full` columns of out_w output columns each and, when the grid's
-// last columns fit in half a wavefront row (128 - 2 hx of them), one split column. A split
-// workgroup steps two half-width tiles of that column, row tiles 2j (lanes 0-31) and 2j + 1
-// (lanes 32-63), so the narrow rest of the grid takes half as many workgroups.
-struct BlockCols {
-  int full;
-  bool split;
-};
-template <class S>
-__host__ __device__ inline BlockCols block_cols(int D2) {
-  const int full = D2 / S::out_w, rest = D2 - full * S::out_w;
-  if (rest == 0) return {full, false};
-  if (rest <= 128 - 2 * S::hx) return {full, true};
-  return {full + 1, false};
-}
-__host__ __device__ inline int block_tiles(BlockCols c, int nY, int nG) {
-  return (c.split ? (nY + 1) / 2 * (2 * c.full + 1) : c.full * nY) * nG;
-}
-// Tile of this workgroup for block_cols' layout, per shot group; the XCD map as in tile_of.
-// With a split column, each pair of row tiles lists its full tiles and then its split tile,
-// so the split workgroups are spread over the launch rather than left to its end (for an odd
-// number of row tiles, the last pair's second row of full tiles is listed and returns).
-__device__ __forceinline__ Tile tile_of_cols(BlockCols c, int nY, int nG, int xcd_map) {
-  const int per_grp = block_tiles(c, nY, 1);
-  const int T = per_grp * nG;
-  const int per = (T + kXcds - 1) / kXcds;
-  const int i = blockIdx.x;
-  const int t = xcd_map ? (i % kXcds) * per + i / kXcds : i;
-  Tile r;
-  r.grp = t / per_grp;
-  const int u = t - r.grp * per_grp;
-  if (!c.split) {
-    r.cx = u % c.full;
-    r.ry = u / c.full;
-  } else {
-    const int pair = u / (2 * c.full + 1), v = u - pair * (2 * c.full + 1);
-    if (v < 2 * c.full) {
-      r.cx = v % c.full;
-      r.ry = 2 * pair + v / c.full;
-    } else {
-      r.cx = c.full;
-      r.ry = 2 * pair;
-    }
-  }
-  r.valid = t < T && r.ry < nY;
-  return r;
-}
 
 typedef __attribute__((address_space(1))) void gmem_void;
 typedef __attribute__((address_space(3))) void lds_void;
@@ -958,17 +1006,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           const int c = c0 + k;
           if (lap_out && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
             st4(lap_out + base + (y * P2 + x), lap);
-          float hy = 0.f;
-          if constexpr (ABS != NW_NONE) {
-            if constexpr (SP) {  // both tiles' rows by scalar loads (clamped), picked per lane
-              const int ya = min(max(ryA + c0 + k, 0), D1 - 1);
-              const int yb = min(max(ryA + TY + c0 + k, 0), D1 - 1);
-              const float da = g.damp[1][ya], db = g.damp[1][yb];
-              hy = lane >= 32 ? db : da;
-            } else {
-              hy = g.damp[1][y];
-            }
-          }
+          const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
           const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
           const float4 pc = cur[k];
           o = make_float4(rhs1<ABS>(pc.x, pv[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
@@ -2146,9 +2184,11 @@ static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots,
                        dim3(grid_tiles3(g, pml_rows<TY>(), shots, spw)), dim3(256), 0, st, g, io,
                        shots, spw, xm & 1);
   } else if ((xm & 4) && TY >= 8 && TY <= 32) {
+    const int tiles = block_tiles(adj_cols((int)g.D[2]), div_up((int)g.D[1], lds_rows<TY>()),
+                                  div_up(shots, spw));
     hipLaunchKernelGGL((k3_adjoint<R, ABS, lds_rows<TY>()>),
-                       dim3(grid_tiles3(g, TY, shots, spw)), dim3(256), 0, st, g, io, shots, spw,
-                       xm & 1);
+                       dim3((unsigned)(kXcds * div_up(tiles, kXcds))), dim3(256), 0, st, g, io,
+                       shots, spw, xm & 1);
   } else {
     hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);

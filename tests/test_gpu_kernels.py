@@ -174,6 +174,14 @@ def test_time_blocking_split_column_is_bit_identical(absorbing, order, n):
     rows = sum(len(r) for _, r in geo)
     w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
                     generator=torch.Generator(DEV).manual_seed(22))
+    # the LDS reverse step splits the columns past the last full 256 the same way (344, 100
+    # and 612 columns): its gradient against the generic kernels'
+    yg, gg = _grad(op, c, w, tiles=[(0, 1), (0, 1)])
+    for rows_adj, spw_adj in ((16, 2), (8, 3), (32, 1)):
+        y, g = _grad(op, c, w, tiles=[(16, 1), (rows_adj, spw_adj)], lds=(1, 1))
+        print(f"split reverse rows {rows_adj} spw {spw_adj}: traces {rel(y, yg):.1e}, "
+              f"gradient {rel(g, gg):.1e}", flush=True)
+        assert rel(y, yg) < 1e-6 and rel(g, gg) < 1e-5
     for ty, spw, depth, xcd in ((32, 4, 4, 1), (16, 1, 4, 0), (8, 3, 3, 1), (32, 2, 2, 1),
                                 (16, 4, 2, 0)):
         tiles = [(ty, spw), (16, 2)]
