@@ -612,6 +612,8 @@ def main() -> int:
         print(f"[bench] kernel-timing step {i} queued", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed_eager = time.perf_counter() - t0
+    if next(iter(op._cache.values())).block != args.ckpt:
+        raise SystemExit("the kernel-timing run changed the snapshot interval")
     timing = {k: nat.profile_read(v) for k, v in kinds.items()}
     units = {k: nat.profile_units(v) for k, v in kinds.items()}
     nat.profile_enable(False)
@@ -632,6 +634,9 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     print(f"[bench] production path: {elapsed * 1e3 / args.steps:.1f} ms per gradient, "
           f"{plan.option(nat.OPT_GRAPH_COUNT)} graphs", file=sys.stderr, flush=True)
+    block = next(iter(op._cache.values())).block
+    if block != args.ckpt:  # the workload snapshots every --ckpt steps; anything else is invalid
+        raise SystemExit(f"snapshot interval {block} used instead of {args.ckpt}")
     ms_per_step = elapsed * 1e3 / args.steps
     value = updates / (elapsed / args.steps) / 1e9
 

@@ -318,13 +318,15 @@ class DevicePlan:
         return prev, cur, faces
 
 
-def grad_block(plan: DevicePlan, n_steps: int, mode: str, every: int | None) -> int:
+def grad_block(plan: DevicePlan, n_steps: int, mode: str, every: int | None,
+               held: int = 0) -> int:
     """Snapshot interval K of the reverse sweep.
 
     ``autograd`` keeps every step of the forward sweep (no recomputation); ``checkpoint`` and
     ``adjoint`` keep the state every K steps (``checkpoint_every`` or ``ceil(sqrt(n_steps))``)
     and recompute one block at a time. A keep buffer that would not fit in free device memory
-    falls back to ``ceil(sqrt(n_steps))``.
+    (plus ``held``, the bytes of the caller's keep buffer, which it reuses) falls back to
+    ``ceil(sqrt(n_steps))``.
     """
     n = max(1, n_steps)
     k = n if mode == "autograd" else int(every or math.ceil(math.sqrt(n)))
@@ -332,7 +334,7 @@ def grad_block(plan: DevicePlan, n_steps: int, mode: str, every: int | None) -> 
     if plan.device.type == "cuda":
         free, _ = torch.cuda.mem_get_info(plan.device)
         need = (k + 1) * plan.keep_slot * plan.esize
-        if need > 0.6 * free:
+        if need > 0.6 * (free + held):
             k = max(1, min(k, math.ceil(math.sqrt(n))))
     return k
 
@@ -401,6 +403,7 @@ class Propagator:
         self.grad_mode = grad_mode
         self.checkpoint_every = checkpoint_every
         self.ws = Workspace(plan)
+        self.block: int | None = None  # snapshot interval of the last gradient forward sweep
 
     def _workspace(self, hold: bool) -> Workspace:
         if self.ws.busy:
@@ -425,22 +428,25 @@ class Propagator:
         if not hold:
             P.forward(m, state, psi, 0, N, traces=traces)
             return traces.clone(), None
-        K = grad_block(P, N, self.grad_mode, self.checkpoint_every)
+        # the workspace's keep buffers are reused, so they count as available memory
+        held = sum(t.numel() * t.element_size() for k, t in ws.bufs.items()
+                   if k.startswith("keep"))
+        K = grad_block(P, N, self.grad_mode, self.checkpoint_every, held)
+        self.block = K
         nb = -(-N // K)
         last0 = (nb - 1) * K
         ckpt = ws.get("ckpt", (nb - 1) * P.ckpt_slot, zero_new=True) if nb > 1 else None
-        # with room for the overlapped reverse sweep's second keep buffer, this sweep keeps the
-        # last two blocks, so the reverse sweep recomputes one block fewer
-        two = nb > 2 and self._overlap_fits(ws, K)
-        kept0 = last0 - K if two else last0
+        # with room for the overlapped reverse sweep's keep buffers, this sweep keeps the last
+        # J blocks (one per buffer), so the reverse sweep recomputes J - 1 blocks fewer
+        J = self._kept_blocks(ws, K, nb)
+        kept0 = (nb - J) * K
         if kept0 > 0:
             P.forward(m, state, psi, 0, kept0, traces=traces, ckpt=ckpt, ckpt_every=K)
-        if two:
-            P.forward(m, state, psi, kept0, last0, traces=traces,
-                      keep=ws.get("keep2", (K + 1) * P.keep_slot, zero_new=True))
-        keep = ws.get("keep", (K + 1) * P.keep_slot, zero_new=True)
-        P.forward(m, state, psi, last0, N, traces=traces, keep=keep)
-        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, two,
+        for b in range(nb - J, nb):
+            P.forward(m, state, psi, b * K, min(N, (b + 1) * K), traces=traces,
+                      keep=self._keep_buf(ws, K, nb - 1 - b))
+        keep = self._keep_buf(ws, K, 0)
+        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, J,
                                 _Lease(ws) if ws is self.ws else None)
 
     def fwi_backward(self, c: torch.Tensor, saved, residual: torch.Tensor) -> torch.Tensor:
@@ -450,16 +456,16 @@ class Propagator:
         grad_c = P.zeros(nf)
         if saved is None:
             return grad_c
-        m, ma, ckpt, keep, K, nb, ws, kept2, _lease = saved
+        m, ma, ckpt, keep, K, nb, ws, kept, _lease = saved
         N = h.n_steps
         res = ws.get("residual", residual.numel())
         res.copy_(residual.reshape(-1))
         lam, psib = ws.state("lam")
         grad_m = ws.get("grad_m", P.grad_slices() * h.cells, torch.float64, zero=True)
         state, psi = ws.state("s")
-        if nb > 2 and self._overlap_fits(ws, K):
-            self._backward_overlapped(m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m,
-                                      state, psi, kept2)
+        if kept > 1 or (nb > 2 and self._overlap_fits(ws, K)):
+            self._backward_overlapped(m, ma, ckpt, K, nb, ws, res, lam, psib, grad_m,
+                                      state, psi, kept)
             nb = 0  # done
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
@@ -498,21 +504,51 @@ class Propagator:
         free, _ = torch.cuda.mem_get_info(P.device)
         return (K + 1) * P.keep_slot * P.esize < 0.8 * free
 
-    def _backward_overlapped(self, m, ma, ckpt, keep, K, nb, ws, res, lam, psib, grad_m, state,
-                             psi, kept2: bool) -> None:
-        """The reverse sweep with block b-1 recomputed on a second stream while the reverse
-        steps of block b run: two keep buffers alternate between blocks; a recompute waits
-        until the reverse steps that read its buffer have finished, and the reverse steps of a
-        block wait for its recompute. ``kept2``: the forward sweep already filled the second
-        buffer with block nb-2."""
+    @staticmethod
+    def _keep_buf(ws: Workspace, K: int, j: int) -> torch.Tensor:
+        """Keep buffer j (block nb-1-j of the forward sweep, then every len(bufs)-th block
+        before it in the reverse sweep)."""
+        P = ws.plan
+        return ws.get("keep" if j == 0 else f"keep{j + 1}", (K + 1) * P.keep_slot, zero_new=True)
+
+    def _kept_blocks(self, ws: Workspace, K: int, nb: int) -> int:
+        """Blocks the forward sweep keeps for the reverse sweep, one keep buffer each: 1, or
+        with the overlapped reverse sweep 2 and more while their buffers fit in 85 % of free
+        device memory (at most ``NEFI_AMD_KEEP_BLOCKS``, default 4). Each one more is one
+        block fewer to recompute."""
+        if nb <= 2 or not self._overlap_fits(ws, K):
+            return 1
+        P = self.plan
+        size = (K + 1) * P.keep_slot
+        cap = max(2, min(nb, int(os.environ.get("NEFI_AMD_KEEP_BLOCKS", "4"))))
+        free, _ = torch.cuda.mem_get_info(P.device)
+        budget = 0.85 * free
+        j = 0
+        while j < cap:
+            name = "keep" if j == 0 else f"keep{j + 1}"
+            if not (name in ws.bufs and ws.bufs[name].numel() >= size):
+                if size * P.esize > budget:
+                    break
+                budget -= size * P.esize
+            j += 1
+        return max(2, j)
+
+    def _backward_overlapped(self, m, ma, ckpt, K, nb, ws, res, lam, psib, grad_m, state,
+                             psi, kept: int) -> None:
+        """The reverse sweep with earlier blocks recomputed on a second stream while the
+        reverse steps of block b run: max(kept, 2) keep buffers rotate between blocks (block b
+        reuses the buffer of block b + len(bufs)); a recompute waits until the reverse steps
+        that read its buffer have finished, and the reverse steps of a block wait for its
+        recompute. ``kept``: the forward sweep filled buffer j with block nb-1-j for j < kept."""
         P = self.plan
         N = P.host.n_steps
-        bufs = (keep, ws.get("keep2", (K + 1) * P.keep_slot, zero_new=True))
+        nbuf = max(kept, 2)
+        bufs = [self._keep_buf(ws, K, j) for j in range(nbuf)]
         main = torch.cuda.current_stream(P.device)
         side = ws.side_stream()
 
         def keep_of(b):
-            return bufs[(nb - 1 - b) % 2]
+            return bufs[(nb - 1 - b) % nbuf]
 
         def recompute(b, after: torch.cuda.Event) -> torch.cuda.Event:
             n0, n1 = b * K, min(N, (b + 1) * K)
@@ -530,20 +566,21 @@ class Propagator:
             return done
 
         ready = {}
-        if not kept2:
+        if kept < nbuf:
             start = torch.cuda.Event()
-            start.record(main)  # the forward sweep's snapshots and last block are written
-            ready[nb - 2] = recompute(nb - 2, start)
+            start.record(main)  # the forward sweep's snapshots and kept blocks are written
+            for b in range(nb - 1 - kept, nb - 1 - nbuf, -1):  # blocks whose buffer is free
+                ready[b] = recompute(b, start)
         for b in range(nb - 1, -1, -1):
             n0, n1 = b * K, min(N, (b + 1) * K)
             if b in ready:
                 main.wait_event(ready.pop(b))
             P.adjoint(ma, lam, psib, n0, n1, keep=keep_of(b), residual=res,
                       inject_end=(b == nb - 1), grad=grad_m)
-            if b >= 2:  # block b-2 reuses block b's keep buffer
+            if b >= nbuf:  # block b - nbuf reuses block b's keep buffer
                 read = torch.cuda.Event()
                 read.record(main)
-                ready[b - 2] = recompute(b - 2, read)
+                ready[b - nbuf] = recompute(b - nbuf, read)
 
     # ---- PAT: p0 -> traces -----------------------------------------------------------------
     def ic_forward(self, m: torch.Tensor, p0: torch.Tensor) -> torch.Tensor:

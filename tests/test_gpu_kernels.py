@@ -422,8 +422,9 @@ def test_autograd_comparator_matches_hip_gradient():
 
 @pytest.mark.parametrize("absorbing", ["sponge", "pml"])
 def test_overlapped_reverse_sweep_is_bit_identical(absorbing, monkeypatch):
-    """The reverse sweep with the next block recomputed on a second stream (two keep buffers)
-    gives exactly the gradient of the one-stream sweep, eager and replayed from graphs."""
+    """The reverse sweep with earlier blocks recomputed on a second stream while the forward
+    sweep keeps the last 2, 3 or 4 blocks (as many rotating keep buffers) gives exactly the
+    gradient of the one-stream sweep, eager and replayed from graphs (7 blocks of 13 steps)."""
     from nefi_amd import _native as nat
 
     print(f"[case overlapped reverse sweep {absorbing}]", flush=True)
@@ -432,16 +433,18 @@ def test_overlapped_reverse_sweep_is_bit_identical(absorbing, monkeypatch):
     rows = sum(len(r) for _, r in geo)
     w = torch.randn(rows, kw["n_t"], device=DEV, generator=torch.Generator(DEV).manual_seed(9))
     x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
-    grads = {}
-    for flag in ("0", "1", "1"):
+    grads = []
+    for flag, kept in (("0", "4"), ("1", "2"), ("1", "3"), ("1", "4"), ("1", "4")):
         monkeypatch.setenv("NEFI_AMD_OVERLAP", flag)
+        monkeypatch.setenv("NEFI_AMD_KEEP_BLOCKS", kept)
         (g,) = torch.autograd.grad((op({"c": x}) * w).sum(), x)
         torch.cuda.synchronize()
-        grads.setdefault(flag, []).append(g)
+        grads.append(g)
+        print(f"overlap {flag} kept blocks {kept}: equal {torch.equal(g, grads[0])}", flush=True)
     prop = next(iter(op._cache.values()))
-    assert "keep2" in prop.ws.bufs and prop.plan.option(nat.OPT_GRAPH) in (0, 1)
-    for g in grads["1"]:
-        assert torch.equal(g, grads["0"][0])
+    assert "keep4" in prop.ws.bufs and prop.plan.option(nat.OPT_GRAPH) in (0, 1)
+    for g in grads[1:]:
+        assert torch.equal(g, grads[0])
 
 
 def test_misaligned_buffers_are_refused():
