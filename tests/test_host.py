@@ -245,3 +245,26 @@ def test_shipped_tile_table_holds_valid_choices():
             cands = at.candidates(kind, int(shots[1:]), absorbing == "a3")
             assert (c["rows"], c["shots_per_group"], c["rows_per_iter"], c["lds"],
                     c["depth"]) in cands, (key, name)
+
+
+def test_snapshot_interval_counts_the_held_keep_buffers(monkeypatch):
+    """The gradient's snapshot interval falls back to ceil(sqrt(N)) only when a keep buffer of
+    K + 1 wavefields does not fit in 60 % of the available memory, and the workspace's own keep
+    buffers (reused by the next gradient) count as available. The 64-shot 1064^2 workload with
+    K = 200 (58.25 GB per keep buffer) and four buffers held leaves ~40 GB free: without the
+    held bytes the next gradient would silently snapshot every 64 steps."""
+    from types import SimpleNamespace
+
+    from nefi_amd.physics._engine import grad_block
+
+    plan = SimpleNamespace(device=torch.device("cuda", 0), keep_slot=1064 * 1064 * 64, esize=4)
+    keep = 201 * plan.keep_slot * plan.esize
+    free = 40 * 10**9
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda device=None: (free, 288 * 10**9))
+    assert grad_block(plan, 4000, "checkpoint", 200) == 64  # nothing held: does not fit
+    assert grad_block(plan, 4000, "checkpoint", 200, held=4 * keep) == 200
+    free = 20 * 10**9
+    assert grad_block(plan, 4000, "checkpoint", 200, held=keep) == 64  # 0.6 x 78 GB < 58 GB
+    free = 285 * 10**9
+    assert grad_block(plan, 4000, "checkpoint", 200) == 200
+    assert grad_block(plan, 4000, "autograd", None, held=0) == 64  # store-all does not fit
