@@ -634,9 +634,12 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     print(f"[bench] production path: {elapsed * 1e3 / args.steps:.1f} ms per gradient, "
           f"{plan.option(nat.OPT_GRAPH_COUNT)} graphs", file=sys.stderr, flush=True)
-    block = next(iter(op._cache.values())).block
-    if block != args.ckpt:  # the workload snapshots every --ckpt steps; anything else is invalid
-        raise SystemExit(f"snapshot interval {block} used instead of {args.ckpt}")
+    prop = next(iter(op._cache.values()))
+    if prop.block != args.ckpt:  # the workload snapshots every --ckpt steps; nothing else is valid
+        raise SystemExit(f"snapshot interval {prop.block} used instead of {args.ckpt}")
+    # blocks the forward sweep keeps for the reverse sweep (one keep buffer each)
+    kept_blocks = sum(1 for name in prop.ws.bufs if name.startswith("keep"))
+    del prop  # the comparator later frees the operator's workspace; hold no reference to it
     ms_per_step = elapsed * 1e3 / args.steps
     value = updates / (elapsed / args.steps) / 1e9
 
@@ -728,7 +731,8 @@ def main() -> int:
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 2),
         "path": {"hip_graph": bool(plan.option(nat.OPT_GRAPH)),
-                 "overlapped_reverse_sweep": os.environ.get("NEFI_AMD_OVERLAP", "1") != "0"},
+                 "overlapped_reverse_sweep": os.environ.get("NEFI_AMD_OVERLAP", "1") != "0",
+                 "snapshot_every": args.ckpt, "kept_blocks": kept_blocks},
         "kernel_timing_run": {
             "what": "eager calls, every launch bracketed by HIP events, one-stream reverse "
                     "sweep (the roofline and kernels_per_gradient come from this run)",
