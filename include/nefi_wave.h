@@ -182,7 +182,9 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   range and flags) into a hipGraph on a private stream the first time and replays it on the
  *   caller's stream afterwards; 0 (default) launches the kernels one by one (graphs captured
  *   so far are kept for when it is switched back on). Changing tiles or the other options
- *   drops the captured graphs. The whole time loop of a call is one graph launch.
+ *   drops the captured graphs. The whole time loop of a call is one graph launch. A plan
+ *   keeps at most 256 graphs: capturing one more first destroys the least recently launched
+ *   one, after waiting for its last launch to complete.
  *   While kernel timing (nw_profile_enable) is on, calls run eagerly.
  * NW_OPT_GRAPH_COUNT (read only): graphs the plan currently holds.
  * NW_OPT_TIME_BLOCK: temporal blocking depth T of the fast forward step: 1 (default) one launch

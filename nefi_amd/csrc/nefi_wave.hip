@@ -75,6 +75,8 @@ struct GraphEntry {
   std::vector<uintptr_t> key;
   hipGraphExec_t exec = nullptr;
   std::vector<void*> out;
+  hipEvent_t done = nullptr;  // recorded after the entry's latest launch
+  int64_t last_use = 0;       // plan's launch clock at that launch
 };
 
 // brackets one launch; returns the record index or -1 when timing is off
@@ -567,7 +569,7 @@ struct nw_plan {
   int graph_mode;
   hipStream_t capture_stream;
   std::vector<nw::GraphEntry> graphs;
-  int64_t graph_serial;
+  int64_t graph_serial;        // launch clock of the graph cache (least recently used goes)
   int64_t shot_bytes;          // one wavefield (all shots)
   int64_t face_bytes[3];       // one face field per axis (all shots), 0 for unused axes
 };
@@ -1214,10 +1216,35 @@ static int adjoint_impl(nw_plan* P, const T* m, const T* ma, void** lam, void** 
 // plans alive in the process (for timing reads across their captured graphs)
 std::vector<nw_plan*> g_plans;
 
+// Graphs a plan keeps. One gradient replays about 2 x (blocks) + 1 distinct calls, so 256
+// holds the calls of sweeps up to ~120 snapshot blocks before any is captured twice.
+constexpr size_t kMaxGraphs = 256;
+
+// Destroy one entry after its latest launch has completed (a launch may still be queued on the
+// caller's stream).
+static void destroy_entry(GraphEntry& e) {
+  if (e.done) {
+    (void)hipEventSynchronize(e.done);  // cleanup: nothing to report
+    (void)hipEventDestroy(e.done);
+  }
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  e.done = nullptr;
+  e.exec = nullptr;
+}
+
 static void drop_graphs(nw_plan* P) {
-  for (auto& e : P->graphs)
-    if (e.exec) (void)hipGraphExecDestroy(e.exec);  // cleanup: nothing to report
+  for (auto& e : P->graphs) destroy_entry(e);
   P->graphs.clear();
+}
+
+// Make room for one more entry by destroying the least recently launched one.
+static void evict_lru(nw_plan* P) {
+  if (P->graphs.size() < kMaxGraphs) return;
+  size_t victim = 0;
+  for (size_t i = 1; i < P->graphs.size(); ++i)
+    if (P->graphs[i].last_use < P->graphs[victim].last_use) victim = i;
+  destroy_entry(P->graphs[victim]);
+  P->graphs.erase(P->graphs.begin() + (std::ptrdiff_t)victim);
 }
 
 // Run `body(stream)` directly, or (graph mode) capture it once per distinct key on the plan's
@@ -1235,7 +1262,7 @@ static int run_captured(nw_plan* P, std::vector<uintptr_t> key, std::vector<void
     if (e.key == key) hit = &e;
   if (!hit) {
     if (!P->capture_stream) NW_HIP(hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking));
-    if (P->graphs.size() >= 64) drop_graphs(P);
+    evict_lru(P);
     NW_HIP(hipStreamBeginCapture(P->capture_stream, hipStreamCaptureModeThreadLocal));
     const int rc = body(P->capture_stream);
     hipGraph_t graph = nullptr;
@@ -1257,6 +1284,9 @@ static int run_captured(nw_plan* P, std::vector<uintptr_t> key, std::vector<void
     hit = &P->graphs.back();
   }
   NW_HIP(hipGraphLaunch(hit->exec, st));
+  if (!hit->done) NW_HIP(hipEventCreateWithFlags(&hit->done, hipEventDisableTiming));
+  NW_HIP(hipEventRecord(hit->done, st));
+  hit->last_use = ++P->graph_serial;
   size_t k = 0;
   for (size_t a = 0; a < io_arrays.size(); ++a)
     for (int i = 0; i < io_sizes[a]; ++i) io_arrays[a][i] = hit->out[k++];

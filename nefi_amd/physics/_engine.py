@@ -352,6 +352,7 @@ class Workspace:
         self.plan = plan
         self.bufs: dict[str, torch.Tensor] = {}
         self.busy = False
+        self.generation = 0  # forward sweeps that saved reverse-sweep state here
         self._side: torch.cuda.Stream | None = None
 
     def side_stream(self) -> torch.cuda.Stream:
@@ -384,15 +385,48 @@ class Workspace:
         faces = [self.get(f"{prefix}f{i}", n, zero=True) for i, n in enumerate(P.face_numel * 2)]
         return st, faces
 
-    def release(self) -> None:
-        self.busy = False
+    def hold(self) -> int:
+        """Mark the workspace held by a new forward sweep; returns that sweep's generation."""
+        self.busy = True
+        self.generation += 1
+        return self.generation
+
+    def release(self, generation: int | None = None) -> None:
+        """End the hold of forward sweep ``generation`` (``None``: whatever holds it). A release
+        for an older sweep leaves a newer sweep's hold in place."""
+        if generation is None or generation == self.generation:
+            self.busy = False
 
 
 class _Lease:
     """Releases a held workspace when the saved reverse-sweep state is dropped."""
 
-    def __init__(self, ws: Workspace) -> None:
-        self._fin = weakref.finalize(self, ws.release)
+    def __init__(self, ws: Workspace, generation: int) -> None:
+        self._fin = weakref.finalize(self, ws.release, generation)
+
+
+@dataclass
+class SavedSweep:
+    """What a forward sweep leaves for its reverse sweep: the model, its snapshots every K
+    steps and the kept blocks, all inside workspace ``ws`` as forward sweep ``generation``
+    wrote them. The first reverse sweep rotates the kept blocks through its own recomputes,
+    so it marks the record ``spent``; a later reverse sweep through the same forward (a second
+    ``autograd.grad`` with ``retain_graph=True``) runs the forward sweep again from ``c``."""
+
+    m: torch.Tensor
+    ma: torch.Tensor
+    ckpt: torch.Tensor | None
+    keep: torch.Tensor
+    K: int
+    nb: int
+    ws: Workspace
+    kept: int
+    generation: int
+    lease: _Lease | None
+    spent: bool = False
+
+    def usable(self) -> bool:
+        return not self.spent and self.ws.generation == self.generation
 
 
 class Propagator:
@@ -405,12 +439,8 @@ class Propagator:
         self.ws = Workspace(plan)
         self.block: int | None = None  # snapshot interval of the last gradient forward sweep
 
-    def _workspace(self, hold: bool) -> Workspace:
-        if self.ws.busy:
-            return Workspace(self.plan)
-        if hold:
-            self.ws.busy = True
-        return self.ws
+    def _workspace(self) -> Workspace:
+        return Workspace(self.plan) if self.ws.busy else self.ws
 
     # ---- FWI: c -> traces ------------------------------------------------------------------
     def fwi_forward(self, c: torch.Tensor, need_grad: bool):
@@ -418,7 +448,8 @@ class Propagator:
         h = P.host
         N = h.n_steps
         hold = need_grad and N > 0
-        ws = self._workspace(hold)
+        ws = self._workspace()
+        gen = ws.hold() if hold else 0
         m = ws.get("m", h.cells)
         ma = ws.get("ma", h.cells) if need_grad else None
         nat.check(nat.lib().nw_model(P.handle, c.data_ptr(), m.data_ptr(), nat.ptr(ma),
@@ -446,17 +477,32 @@ class Propagator:
             P.forward(m, state, psi, b * K, min(N, (b + 1) * K), traces=traces,
                       keep=self._keep_buf(ws, K, nb - 1 - b))
         keep = self._keep_buf(ws, K, 0)
-        return traces.clone(), (m, ma, ckpt, keep, K, nb, ws, J,
-                                _Lease(ws) if ws is self.ws else None)
+        saved = SavedSweep(m, ma, ckpt, keep, K, nb, ws, J, gen,
+                           _Lease(ws, gen) if ws is self.ws else None)
+        return traces.clone(), saved
 
-    def fwi_backward(self, c: torch.Tensor, saved, residual: torch.Tensor) -> torch.Tensor:
+    def fwi_backward(self, c: torch.Tensor, saved: SavedSweep | None,
+                     residual: torch.Tensor) -> torch.Tensor:
+        """dJ/dc for the residual dJ/dtraces. ``saved`` is the record of the forward sweep
+        (None only for a run of zero steps, whose traces do not depend on ``c``). A record
+        already spent by an earlier reverse sweep, or whose workspace a later forward sweep
+        reused, is rebuilt by running the forward sweep from ``c`` again (as
+        ``torch.utils.checkpoint`` recomputes on every backward), so every backward through one
+        forward gives the same gradient."""
         P = self.plan
         h = P.host
         nf = h.field_dims[0] * h.field_dims[1] * h.field_dims[2]
         grad_c = P.zeros(nf)
         if saved is None:
+            if h.n_steps > 0:
+                raise RuntimeError("wave operator: the forward sweep saved no reverse-sweep "
+                                   "state although the input required a gradient")
             return grad_c
-        m, ma, ckpt, keep, K, nb, ws, kept, _lease = saved
+        if not saved.usable():
+            _, saved = self.fwi_forward(c, True)
+        saved.spent = True
+        m, ma, ckpt, keep, K, nb, ws, kept = (saved.m, saved.ma, saved.ckpt, saved.keep,
+                                              saved.K, saved.nb, saved.ws, saved.kept)
         N = h.n_steps
         res = ws.get("residual", residual.numel())
         res.copy_(residual.reshape(-1))
@@ -485,7 +531,7 @@ class Propagator:
                                     P.stream()),
             "nw_model_grad",
         )
-        ws.release()
+        ws.release(saved.generation)
         return grad_c
 
     def _overlap_fits(self, ws: Workspace, K: int) -> bool:
@@ -499,10 +545,13 @@ class Propagator:
         if P.dtype == torch.float32 and P.tiles(1)[0] > 0 and \
                 P.option(nat.OPT_TIME_BLOCK_ADJOINT) > 1:
             return False
-        if "keep2" in ws.bufs and ws.bufs["keep2"].numel() >= (K + 1) * P.keep_slot:
+        size = (K + 1) * P.keep_slot
+        missing = sum(1 for name in ("keep", "keep2")
+                      if not (name in ws.bufs and ws.bufs[name].numel() >= size))
+        if missing == 0:
             return True
         free, _ = torch.cuda.mem_get_info(P.device)
-        return (K + 1) * P.keep_slot * P.esize < 0.8 * free
+        return missing * size * P.esize < 0.8 * free
 
     @staticmethod
     def _keep_buf(ws: Workspace, K: int, j: int) -> torch.Tensor:
@@ -515,7 +564,7 @@ class Propagator:
         """Blocks the forward sweep keeps for the reverse sweep, one keep buffer each: 1, or
         with the overlapped reverse sweep 2 and more while their buffers fit in 85 % of free
         device memory (at most ``NEFI_AMD_KEEP_BLOCKS``, default 4). Each one more is one
-        block fewer to recompute."""
+        block fewer to recompute. When fewer than two buffers fit, 1 (the one-stream sweep)."""
         if nb <= 2 or not self._overlap_fits(ws, K):
             return 1
         P = self.plan
@@ -531,7 +580,7 @@ class Propagator:
                     break
                 budget -= size * P.esize
             j += 1
-        return max(2, j)
+        return j if j >= 2 else 1
 
     def _backward_overlapped(self, m, ma, ckpt, K, nb, ws, res, lam, psib, grad_m, state,
                              psi, kept: int) -> None:
@@ -586,7 +635,7 @@ class Propagator:
     def ic_forward(self, m: torch.Tensor, p0: torch.Tensor) -> torch.Tensor:
         P = self.plan
         h = P.host
-        ws = self._workspace(False)
+        ws = self._workspace()
         state, psi = ws.state()
         nat.check(
             nat.lib().nw_ic_state(P.handle, m.data_ptr(), p0.data_ptr(), state[0].data_ptr(),
@@ -600,7 +649,7 @@ class Propagator:
     def ic_backward(self, m: torch.Tensor, ma: torch.Tensor, residual: torch.Tensor) -> torch.Tensor:
         P = self.plan
         h = P.host
-        ws = self._workspace(False)
+        ws = self._workspace()
         res = ws.get("residual", residual.numel())
         res.copy_(residual.reshape(-1))
         lam, psib = ws.state("lam")
@@ -634,8 +683,12 @@ class FwiFunction(torch.autograd.Function):
         (c,) = ctx.saved_tensors
         prop = ctx.prop
         res = g.to(device=prop.plan.device, dtype=prop.plan.dtype).contiguous()
-        grad = prop.fwi_backward(c.detach().contiguous(), ctx.saved, res)
-        ctx.saved = None
+        saved = ctx.saved
+        grad = prop.fwi_backward(c.detach().contiguous(), saved, res)
+        if saved is not None:
+            # a later backward through this forward reruns the forward sweep from c; keep only
+            # the spent marker, not the workspace's buffers
+            ctx.saved = SavedSweep(*(None,) * 6, saved.ws, 0, saved.generation, None, True)
         return grad.view_as(c), None
 
 

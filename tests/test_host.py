@@ -268,3 +268,38 @@ def test_snapshot_interval_counts_the_held_keep_buffers(monkeypatch):
     free = 285 * 10**9
     assert grad_block(plan, 4000, "checkpoint", 200) == 200
     assert grad_block(plan, 4000, "autograd", None, held=0) == 64  # store-all does not fit
+
+
+def test_kept_blocks_fall_back_to_the_one_stream_sweep_when_two_buffers_do_not_fit(monkeypatch):
+    """The overlapped reverse sweep needs two keep buffers. With 58.25 GB buffers (64 shots of
+    1064^2, K = 200) and ~100 GB free, one fits and two do not: the forward sweep keeps one
+    block and the reverse sweep runs on one stream (it used to allocate a second buffer and
+    run out of memory). Buffers the workspace already holds count as available."""
+    from types import SimpleNamespace
+
+    from nefi_amd.physics._engine import Propagator
+
+    monkeypatch.delenv("NEFI_AMD_OVERLAP", raising=False)
+    monkeypatch.delenv("NEFI_AMD_KEEP_BLOCKS", raising=False)
+    plan = SimpleNamespace(device=torch.device("cuda", 0), keep_slot=1064 * 1064 * 64, esize=4,
+                           dtype=torch.float32, tiles=lambda kind: (16, 8),
+                           option=lambda opt: 1)
+    prop = Propagator.__new__(Propagator)
+    prop.plan = plan
+    size = 201 * plan.keep_slot
+    gb = 10**9
+    free = [0]
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda device=None: (free[0], 288 * gb))
+
+    def kept(free_gb, held=()):
+        free[0] = int(free_gb * gb)
+        ws = SimpleNamespace(plan=plan, bufs={k: SimpleNamespace(numel=lambda: size)
+                                              for k in held})
+        return prop._kept_blocks(ws, 200, 20)
+
+    assert kept(100) == 1 and kept(116) == 1  # one buffer fits, two do not
+    assert kept(150) == 2
+    assert kept(285) == 4
+    assert kept(20, held=("keep", "keep2")) == 2  # the held pair is reused
+    assert kept(20, held=("keep",)) == 1
+    assert kept(30, held=("keep", "keep2", "keep3", "keep4")) == 4
