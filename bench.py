@@ -104,14 +104,16 @@ def cpu_baseline(wl, budget_s=15.0):
     }
 
 
-def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
-    """The plain PyTorch autograd path of the reference on the GPU, for one sample of shots.
+def reference_leapfrog(op, c, shot_ids, n_steps, residual=None, block=200):
+    """The plain PyTorch path of the reference on the GPU, for the shots ``shot_ids``.
 
     The reference's own op sequence for a sponge layer (nefi/physics/wave.py:529-552 inside
     timestep.py:151-169): conv2d Laplacian with the (1,1,5,5) cross kernel, addcmul, a dense
     source map times the host wavelet value, the damping products, multilinear gather at every
-    record step, and torch.utils.checkpoint over blocks of `block` steps. Returns dJ/dc for
-    J = sum(residual * traces).
+    record step, and torch.utils.checkpoint over blocks of `block` steps (block=None: every
+    step's tape kept, grad_mode="autograd"). Returns ``(traces, dJ/dc)`` for
+    J = sum(residual * traces), with ``residual`` the rows of those shots; without a residual
+    it runs under no_grad and returns ``(traces, None)``. ``traces`` is ``(rows, samples)``.
     """
     import torch
     import torch.nn.functional as F
@@ -120,6 +122,8 @@ def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
     g = op.grid
     dev, dt = c.device, op.dt
     dtype = c.dtype
+    shot_ids = list(shot_ids)
+    shots = len(shot_ids)
     kern = torch.zeros(5, 5, dtype=torch.float64)
     for ax, h in enumerate(g.spacing):
         for off, co in ((-2, -1 / 12), (-1, 4 / 3), (0, -2.5), (1, 4 / 3), (2, -1 / 12)):
@@ -131,13 +135,15 @@ def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
     half = 0.5 * sig * dt
     a = (1.0 / (1.0 + half)).to(dev, dtype)
     b = (1.0 - half).to(dev, dtype)
-    src = (g.source_density(torch.cat(op.shot_sources[:shots])) * dt**2).to(dev, dtype)
+    src = (g.source_density(torch.cat([op.shot_sources[s] for s in shot_ids])) * dt**2).to(
+        dev, dtype)
     wav = [float(v) for v in op.wavelet()]
     idx, w = op._rec
-    counts = op.receiver_counts[:shots]
-    rows = sum(counts)
-    ridx = idx[:rows].to(dev)
-    rw = w[:rows].to(dev, dtype)
+    counts = [op.receiver_counts[s] for s in shot_ids]
+    starts = [sum(op.receiver_counts[:s]) for s in shot_ids]
+    rows_sel = torch.cat([torch.arange(r0, r0 + n) for r0, n in zip(starts, counts)])
+    ridx = idx[rows_sel].to(dev)
+    rw = w[rows_sel].to(dev, dtype)
     shot_of = torch.repeat_interleave(torch.arange(shots), torch.tensor(counts)).to(dev)
     flat_idx = shot_of[:, None] * (g.padded_shape[0] * g.padded_shape[1]) + ridx
     record = set(op.record)
@@ -151,31 +157,41 @@ def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
         rhs = rhs + src * wav[n]
         return p, a * rhs
 
-    x = c.detach().clone().requires_grad_(True)
-    c2dt2 = g.pad(x, "replicate") ** 2 * dt**2
-    p_prev = torch.zeros(shots, *g.padded_shape, device=dev, dtype=dtype)
-    p = torch.zeros_like(p_prev)
-    obs = [sample(p)]
-    for n0 in range(0, n_steps, block or n_steps):
-        n1 = min(n_steps, n0 + (block or n_steps))
+    with torch.set_grad_enabled(residual is not None):
+        x = c.detach().clone().requires_grad_(residual is not None)
+        c2dt2 = g.pad(x, "replicate") ** 2 * dt**2
+        p_prev = torch.zeros(shots, *g.padded_shape, device=dev, dtype=dtype)
+        p = torch.zeros_like(p_prev)
+        obs = [sample(p)]
+        blk = block if residual is not None else None
+        for n0 in range(0, n_steps, blk or n_steps):
+            n1 = min(n_steps, n0 + (blk or n_steps))
 
-        def seg(pp, pc, m, n0=n0, n1=n1):
-            out = []
-            for n in range(n0, n1):
-                pp, pc = step(pp, pc, m, n)
-                if n + 1 in record:
-                    out.append(sample(pc))
-            return (pp, pc, *out)
+            def seg(pp, pc, m, n0=n0, n1=n1):
+                out = []
+                for n in range(n0, n1):
+                    pp, pc = step(pp, pc, m, n)
+                    if n + 1 in record:
+                        out.append(sample(pc))
+                return (pp, pc, *out)
 
-        # block=None: plain autograd, every step's tape kept (grad_mode="autograd")
-        res = seg(p_prev, p, c2dt2) if not block else checkpoint(seg, p_prev, p, c2dt2,
-                                                                 use_reentrant=False)
-        p_prev, p = res[0], res[1]
-        obs.extend(res[2:])
-    traces = torch.stack(obs, -1)
-    loss = (traces * residual[:rows, : traces.shape[-1]]).sum()
-    (grad,) = torch.autograd.grad(loss, x)
-    return grad
+            res = seg(p_prev, p, c2dt2) if not blk else checkpoint(seg, p_prev, p, c2dt2,
+                                                                   use_reentrant=False)
+            p_prev, p = res[0], res[1]
+            obs.extend(res[2:])
+        traces = torch.stack(obs, -1)
+        if residual is None:
+            return traces, None
+        loss = (traces * residual[:, : traces.shape[-1]]).sum()
+        (grad,) = torch.autograd.grad(loss, x)
+    return traces.detach(), grad
+
+
+def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
+    """dJ/dc of the reference's plain PyTorch path for the first ``shots`` shots
+    (J = sum(residual * traces); ``residual`` holds at least those shots' rows)."""
+    rows = sum(op.receiver_counts[:shots])
+    return reference_leapfrog(op, c, range(shots), n_steps, residual[:rows], block)[1]
 
 
 def autograd_comparator(op, c, residual, steps=200):

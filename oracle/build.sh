@@ -3,4 +3,4 @@
 set -e
 here="$(cd "$(dirname "$0")" && pwd)"
 mkdir -p "$here/_build"
-gcc -O2 -fPIC -shared -fopenmp -std=c99 -Wall -o "$here/_build/liboracle64.so" "$here/leapfrog64.c"
+gcc -O3 -fPIC -shared -fopenmp -std=c99 -Wall -o "$here/_build/liboracle64.so" "$here/leapfrog64.c"

@@ -66,8 +66,9 @@ def lib():
         vp = c_void_p
         h.or_forward.argtypes = [POINTER(_Prob), vp, vp, vp, vp, vp]
         h.or_gradient.argtypes = [POINTER(_Prob), vp, vp, vp, vp, vp, vp, vp]
+        h.or_gradient_ckpt.argtypes = [POINTER(_Prob), vp, vp, vp, vp, vp, vp, vp, c_int, vp]
         h.or_laplacian.argtypes = [POINTER(_Prob), vp, vp]
-        h.or_forward.restype = h.or_gradient.restype = c_int
+        h.or_forward.restype = h.or_gradient.restype = h.or_gradient_ckpt.restype = c_int
         _lib = h
     return _lib
 
@@ -289,8 +290,13 @@ class Problem:
         assert rc == 0
         return (tr, st) if final_state else tr
 
-    def fwi_gradient(self, c, residual):
-        """Gradient of ``sum(residual * traces(c))`` with respect to ``c``."""
+    def fwi_gradient(self, c, residual, every=None, with_traces=False):
+        """Gradient of ``sum(residual * traces(c))`` with respect to ``c``.
+
+        ``every``: keep the forward state every ``every`` steps and recompute each block before
+        its reverse steps (host memory ~ (every + 1) states instead of n_steps + 1); None keeps
+        every state. Both give identical bits. ``with_traces``: return ``(gradient, traces)``,
+        the traces observed on the recomputed states (the bits of ``fwi_traces``)."""
         S = self.sources.shape[0]
         src = np.ascontiguousarray(self.source_maps())
         c = np.asarray(c, np.float64)
@@ -299,10 +305,13 @@ class Problem:
         res = np.ascontiguousarray(residual, dtype=np.float64)
         gm = np.zeros(self.cells)
         P = self._struct(S, src)
-        rc = lib().or_gradient(ctypes.byref(P), m.ctypes.data, z.ctypes.data, z.ctypes.data,
-                               res.ctypes.data, gm.ctypes.data, None, None)
+        tr = np.zeros((S, self.rec_idx.shape[0], self.n_t)) if with_traces else None
+        rc = lib().or_gradient_ckpt(ctypes.byref(P), m.ctypes.data, z.ctypes.data, z.ctypes.data,
+                                    res.ctypes.data, gm.ctypes.data, None, None,
+                                    int(every or self.n_steps), _p(tr))
         assert rc == 0
-        return 2.0 * self.dt**2 * c * self.fold(gm)
+        g = 2.0 * self.dt**2 * c * self.fold(gm)
+        return (g, tr) if with_traces else g
 
     # ---- initial-value problem ----------------------------------------------------------------
     def ic_state(self, c, p0):

@@ -23,7 +23,9 @@
  * sequence by hand, op by op in reverse order, over the stored forward states; it is written
  * independently of the fused reverse kernel it is used to check.
  *
- * Build: oracle/build.sh (gcc, optional OpenMP over grid rows).
+ * Build: oracle/build.sh (gcc, OpenMP over grid rows and cells; every parallel loop writes
+ * disjoint elements and sums nothing across them, so results do not depend on the thread
+ * count).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -69,39 +71,51 @@ static void coeffs(int order, double c[3]) {
   }
 }
 
-/* out = K * p (zero values outside the grid); K is symmetric so this is also its transpose */
+/* out = K * p (zero values outside the grid); K is symmetric so this is also its transpose.
+ * Per cell: out = centre * p + acc, where centre = sum over used axes a of c0/h_a^2 and acc
+ * adds, axis by axis (0, 1, 2) and radius by radius (1, 2), (c_k/h_a^2) * (p[-k] + p[+k]) with
+ * neighbours outside the grid left out. */
 void or_laplacian(const or_problem* P, const double* p, double* out) {
   const long D0 = P->dims[0], D1 = P->dims[1], D2 = P->dims[2];
   double c[3];
   coeffs(P->order, c);
   const int r = P->order == 4 ? 2 : 1;
+  double centre = 0.0, ck[3][3] = {{0}};
+  for (int a = 0; a < 3; ++a) {
+    if (!used(P, a)) continue;
+    const double ih2 = 1.0 / (P->h[a] * P->h[a]);
+    centre += c[0] * ih2;
+    for (int k = 1; k <= r; ++k) ck[a][k] = c[k] * ih2;
+  }
+  const long st[3] = {D1 * D2, D2, 1};
+  const long dims[3] = {D0, D1, D2};
+  const int a0 = 3 - P->ndim; /* first used axis */
 #pragma omp parallel for collapse(2) schedule(static)
   for (long z = 0; z < D0; ++z)
-    for (long y = 0; y < D1; ++y)
+    for (long y = 0; y < D1; ++y) {
+      const long co_zy[2] = {z, y};
       for (long x = 0; x < D2; ++x) {
-        const long co[3] = {z, y, x};
         const long i = (z * D1 + y) * D2 + x;
-        double centre = 0.0, acc = 0.0;
-        for (int a = 0; a < 3; ++a) {
-          if (!used(P, a)) continue;
-          const double ih2 = 1.0 / (P->h[a] * P->h[a]);
-          centre += c[0] * ih2;
-          const long st = a == 0 ? D1 * D2 : (a == 1 ? D2 : 1);
+        double acc = 0.0;
+        for (int a = a0; a < 3; ++a) {
+          const long co = a == 2 ? x : co_zy[a];
           for (int k = 1; k <= r; ++k) {
             double s = 0.0;
-            if (co[a] - k >= 0) s += p[i - k * st];
-            if (co[a] + k < P->dims[a]) s += p[i + k * st];
-            acc += c[k] * ih2 * s;
+            if (co - k >= 0) s += p[i - k * st[a]];
+            if (co + k < dims[a]) s += p[i + k * st[a]];
+            acc += ck[a][k] * s;
           }
         }
         out[i] = centre * p[i] + acc;
       }
+    }
 }
 
 /* fd_f = p[f] - p[f-1] along axis a on the dims[a]+1 faces, zero outside */
 static void face_diff(const or_problem* P, int a, const double* p, double* fd) {
   const long E0 = P->dims[0] + (a == 0), E1 = P->dims[1] + (a == 1), E2 = P->dims[2] + (a == 2);
   const long D1 = P->dims[1], D2 = P->dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
   for (long z = 0; z < E0; ++z)
     for (long y = 0; y < E1; ++y)
       for (long x = 0; x < E2; ++x) {
@@ -120,6 +134,7 @@ static void face_diff(const or_problem* P, int a, const double* p, double* fd) {
 static void diff_faces(const or_problem* P, int a, const double* q, double* out) {
   const long D0 = P->dims[0], D1 = P->dims[1], D2 = P->dims[2];
   const long E1 = D1 + (a == 1), E2 = D2 + (a == 2);
+#pragma omp parallel for collapse(2) schedule(static)
   for (long z = 0; z < D0; ++z)
     for (long y = 0; y < D1; ++y)
       for (long x = 0; x < D2; ++x) {
@@ -136,6 +151,7 @@ static void face_diff_T(const or_problem* P, int a, const double* g, double* out
   /* fd_f = p_f - p_{f-1}  =>  out_i = g_i - g_{i+1} */
   const long D0 = P->dims[0], D1 = P->dims[1], D2 = P->dims[2];
   const long E1 = D1 + (a == 1), E2 = D2 + (a == 2);
+#pragma omp parallel for collapse(2) schedule(static)
   for (long z = 0; z < D0; ++z)
     for (long y = 0; y < D1; ++y)
       for (long x = 0; x < D2; ++x) {
@@ -151,6 +167,7 @@ static void diff_faces_T(const or_problem* P, int a, const double* g, double* ou
   /* out_i = q_{i+1} - q_i  =>  qbar_f = g_{f-1} - g_f (zero outside) */
   const long E0 = P->dims[0] + (a == 0), E1 = P->dims[1] + (a == 1), E2 = P->dims[2] + (a == 2);
   const long D1 = P->dims[1], D2 = P->dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
   for (long z = 0; z < E0; ++z)
     for (long y = 0; y < E1; ++y)
       for (long x = 0; x < E2; ++x) {
@@ -188,9 +205,13 @@ static void step(const or_problem* P, const double* m, int s, int n, const doubl
       if (!used(P, a)) continue;
       const long F = fcells_of(P, a);
       face_diff(P, a, cur, tmpf);
-      for (long f = 0; f < F; ++f) new_psi[a][f] = P->alpha[a][f] * psi[a][f] + P->beta[a][f] * tmpf[f];
-      for (long f = 0; f < F; ++f) tmpf[f] = new_psi[a][f] + psi[a][f];
+#pragma omp parallel for schedule(static)
+      for (long f = 0; f < F; ++f) {
+        new_psi[a][f] = P->alpha[a][f] * psi[a][f] + P->beta[a][f] * tmpf[f];
+        tmpf[f] = new_psi[a][f] + psi[a][f];
+      }
       diff_faces(P, a, tmpf, tmpc);
+#pragma omp parallel for schedule(static)
       for (long i = 0; i < N; ++i) lap[i] += tmpc[i] * (0.5 / P->h[a]);
     }
   const double w = (P->src && n < P->n_steps) ? P->wavelet[n] : 0.0;
@@ -258,12 +279,24 @@ int or_forward(const or_problem* P, const double* m, const double* p_prev0, cons
 
 /* Gradient of J = sum(residual * traces) with respect to m (summed over shots, (cells)) and
  * to the initial state of every shot (grad_prev0, grad_p0: (S, cells); either may be NULL).
- * Stores every forward state, then runs the reverse of step() op by op. */
-int or_gradient(const or_problem* P, const double* m, const double* p_prev0, const double* p0,
-                const double* residual, double* grad_m, double* grad_prev0, double* grad_p0) {
+ *
+ * The reverse of step() is run op by op over stored forward states. With every >= n_steps all
+ * states are stored at once; otherwise the forward states at steps 0, every, 2*every, ... are
+ * kept and each block of `every` steps is recomputed from its snapshot before its reverse
+ * steps (the scheme of torch.utils.checkpoint in nefi/physics/timestep.py:151-169). The
+ * recomputation repeats the same operations on the same inputs, so both give identical bits.
+ * The reverse step reads the Laplacian (PML face terms included) that the recomputed forward
+ * step formed, instead of forming it again. If `traces` is not NULL it receives the traces
+ * (S, R, n_t), observed on the recomputed states.
+ */
+int or_gradient_ckpt(const or_problem* P, const double* m, const double* p_prev0,
+                     const double* p0, const double* residual, double* grad_m,
+                     double* grad_prev0, double* grad_p0, int every, double* traces) {
   const long N = cells_of(P);
   const int T = P->n_steps;
   const int pml = P->absorbing == 3;
+  if (every <= 0 || every > T) every = T > 0 ? T : 1;
+  const int nb = T > 0 ? (T + every - 1) / every : 1;
   long F[3] = {0, 0, 0}, Ftot = 0;
   for (int a = 0; a < 3; ++a)
     if (pml && used(P, a)) {
@@ -271,13 +304,18 @@ int or_gradient(const or_problem* P, const double* m, const double* p_prev0, con
       Ftot += F[a];
     }
   const long slot = 2 * N + Ftot; /* prev, cur, psi */
-  double* hist = (double*)malloc((size_t)(T + 1) * slot * sizeof(double));
+  /* hist: the states of one block; snap: the state at the start of every block */
+  double* hist = (double*)malloc((size_t)(every + 1) * slot * sizeof(double));
+  double* snap = (double*)malloc((size_t)nb * slot * sizeof(double));
+  double* laps = (double*)malloc((size_t)every * N * sizeof(double)); /* lap of each step */
   long Fmax = 1;
   for (int a = 0; a < 3; ++a)
     if (fcells_of(P, a) > Fmax) Fmax = fcells_of(P, a);
   double* work = (double*)calloc((size_t)(10 * N + 8 * Fmax), sizeof(double));
-  if (!hist || !work) {
+  if (!hist || !snap || !laps || !work) {
     free(hist);
+    free(snap);
+    free(laps);
     free(work);
     return 1;
   }
@@ -299,27 +337,37 @@ int or_gradient(const or_problem* P, const double* m, const double* p_prev0, con
     ngpsi[a] = tmpf + (3 + 2 * a) * Fmax;
   }
   memset(grad_m, 0, N * sizeof(double));
+  /* state slots hold (prev, cur, psi of the used axes in order) */
+#define SLOT_PSI(base_, psi_)                                    \
+  do {                                                           \
+    long off_ = 2 * N;                                           \
+    for (int a_ = 0; a_ < 3; ++a_) {                             \
+      (psi_)[a_] = 0;                                            \
+      if (F[a_]) {                                               \
+        (psi_)[a_] = (base_) + off_;                             \
+        off_ += F[a_];                                           \
+      }                                                          \
+    }                                                            \
+  } while (0)
   for (int s = 0; s < P->n_shots; ++s) {
-    /* forward, storing state n in hist[n] */
-    double* h0 = hist;
-    memcpy(h0, p_prev0 + (long)s * N, N * sizeof(double));
-    memcpy(h0 + N, p0 + (long)s * N, N * sizeof(double));
-    memset(h0 + 2 * N, 0, Ftot * sizeof(double));
-    for (int n = 0; n < T; ++n) {
-      double* hn = hist + (long)n * slot;
-      double* hx = hist + (long)(n + 1) * slot;
-      double* psi[3] = {0, 0, 0};
-      double* npsi[3] = {0, 0, 0};
-      long off = 2 * N;
-      for (int a = 0; a < 3; ++a)
-        if (F[a]) {
-          psi[a] = hn + off;
-          npsi[a] = hx + off;
-          off += F[a];
-        }
-      step(P, m, s, n, hn, hn + N, psi, hx, hx + N, npsi, lap, tmpf, tmpc);
+    /* forward sweep to the start of the last block, keeping every block's first state */
+    memcpy(snap, p_prev0 + (long)s * N, N * sizeof(double));
+    memcpy(snap + N, p0 + (long)s * N, N * sizeof(double));
+    memset(snap + 2 * N, 0, Ftot * sizeof(double));
+    for (int b = 1; b < nb; ++b) {
+      memcpy(hist, snap + (long)(b - 1) * slot, slot * sizeof(double));
+      for (int n = (b - 1) * every; n < b * every; ++n) {
+        double* hn = hist + (long)((n - (b - 1) * every) & 1) * slot;
+        double* hx = hist + (long)(((n - (b - 1) * every) + 1) & 1) * slot;
+        double* psi[3];
+        double* npsi[3];
+        SLOT_PSI(hn, psi);
+        SLOT_PSI(hx, npsi);
+        step(P, m, s, n, hn, hn + N, psi, hx, hx + N, npsi, lap, tmpf, tmpc);
+      }
+      memcpy(snap + (long)b * slot, hist + (long)(every & 1) * slot, slot * sizeof(double));
     }
-    /* reverse sweep */
+    /* reverse sweep, block by block */
     memset(gprev, 0, N * sizeof(double));
     memset(gcur, 0, N * sizeof(double));
     for (int a = 0; a < 3; ++a)
@@ -333,73 +381,88 @@ int or_gradient(const or_problem* P, const double* m, const double* p_prev0, con
         (g_)[P->rec_idx[r * P->rec_taps + k]] += P->rec_w[r * P->rec_taps + k] * res[r * P->n_t + j_]; \
   }
     INJECT(T, gcur);
-    for (int n = T - 1; n >= 0; --n) {
-      const double* hn = hist + (long)n * slot;
-      const double* cur = hn + N;
-      const double* psi[3] = {0, 0, 0};
-      const double* npsi[3] = {0, 0, 0};
-      long off = 2 * N;
-      for (int a = 0; a < 3; ++a)
-        if (F[a]) {
-          psi[a] = hn + off;
-          npsi[a] = hist + (long)(n + 1) * slot + off;
-          off += F[a];
-        }
-      /* recompute lap^n of the forward step */
-      or_laplacian(P, cur, lap);
-      if (pml)
-        for (int a = 0; a < 3; ++a) {
-          if (!F[a]) continue;
-          for (long f = 0; f < F[a]; ++f) tmpf[f] = npsi[a][f] + psi[a][f];
-          diff_faces(P, a, tmpf, tmpc);
-          for (long i = 0; i < N; ++i) lap[i] += tmpc[i] * (0.5 / P->h[a]);
-        }
-      /* outputs: (new_prev, new_cur[, new_psi]) with adjoints (gprev, gcur[, gpsi]) */
-      memset(nprev_g, 0, N * sizeof(double));
-      memset(ncur_g, 0, N * sizeof(double));
-      for (long i = 0; i < N; ++i) {
-        double gn = gcur[i];  /* adjoint of nxt (before taper) */
-        double gc_direct = gprev[i];
-        if (P->taper) {
-          gn = P->taper[i] * gcur[i];
-          gc_direct = P->taper[i] * gprev[i];
-        }
-        const double gr = P->a ? P->a[i] * gn : gn; /* rhs adjoint */
-        rhsbar[i] = gr;
-        /* rhs = 2 cur - b prev + m lap - zz cur + src w */
-        ncur_g[i] = gc_direct + 2.0 * gr - (P->zz ? P->zz[i] * gr : 0.0);
-        nprev_g[i] = -(P->b ? P->b[i] * gr : gr);
-        grad_m[i] += gr * lap[i];
-        lapbar[i] = m[i] * gr;
+    for (int b = nb - 1; b >= 0; --b) {
+      const int n0 = b * every, n1 = (b + 1) * every < T ? (b + 1) * every : T;
+      /* recompute the block's states: hist[k] = state n0 + k */
+      memcpy(hist, snap + (long)b * slot, slot * sizeof(double));
+      if (traces && n0 == 0 && recorded(P, 0)) observe(P, hist + N, s, 0, traces);
+      for (int n = n0; n < n1; ++n) {
+        double* hn = hist + (long)(n - n0) * slot;
+        double* hx = hist + (long)(n - n0 + 1) * slot;
+        double* psi[3];
+        double* npsi[3];
+        SLOT_PSI(hn, psi);
+        SLOT_PSI(hx, npsi);
+        step(P, m, s, n, hn, hn + N, psi, hx, hx + N, npsi, laps + (long)(n - n0) * N, tmpf,
+             tmpc);
+        if (traces && recorded(P, n + 1)) observe(P, hx + N, s, (n + 1) / P->substeps, traces);
       }
-      /* lap = K cur + sum_a 0.5/h diff(npsi + psi) */
-      or_laplacian(P, lapbar, tmpc2);
-      for (long i = 0; i < N; ++i) ncur_g[i] += tmpc2[i];
-      if (pml)
-        for (int a = 0; a < 3; ++a) {
-          if (!F[a]) continue;
-          /* sbar = diff^T(lapbar) * 0.5/h feeds both npsi and psi */
-          diff_faces_T(P, a, lapbar, tmpf);
-          for (long f = 0; f < F[a]; ++f) tmpf[f] *= 0.5 / P->h[a];
-          /* total adjoint of npsi: from later steps (gpsi) + this term */
-          for (long f = 0; f < F[a]; ++f) tmpf2[f] = gpsi[a][f] + tmpf[f];
-          /* npsi = alpha psi + beta fd(cur) */
-          for (long f = 0; f < F[a]; ++f) ngpsi[a][f] = tmpf[f] + P->alpha[a][f] * tmpf2[f];
-          for (long f = 0; f < F[a]; ++f) tmpf[f] = P->beta[a][f] * tmpf2[f];
-          face_diff_T(P, a, tmpf, tmpc);
-          for (long i = 0; i < N; ++i) ncur_g[i] += tmpc[i];
+      for (int n = n1 - 1; n >= n0; --n) {
+        /* lap^n of the forward step n, as the recomputation formed it */
+        const double* lapn = laps + (long)(n - n0) * N;
+        /* outputs: (new_prev, new_cur[, new_psi]) with adjoints (gprev, gcur[, gpsi]) */
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < N; ++i) {
+          double gn = gcur[i];  /* adjoint of nxt (before taper) */
+          double gc_direct = gprev[i];
+          if (P->taper) {
+            gn = P->taper[i] * gcur[i];
+            gc_direct = P->taper[i] * gprev[i];
+          }
+          const double gr = P->a ? P->a[i] * gn : gn; /* rhs adjoint */
+          rhsbar[i] = gr;
+          /* rhs = 2 cur - b prev + m lap - zz cur + src w */
+          ncur_g[i] = gc_direct + 2.0 * gr - (P->zz ? P->zz[i] * gr : 0.0);
+          nprev_g[i] = -(P->b ? P->b[i] * gr : gr);
+          grad_m[i] += gr * lapn[i];
+          lapbar[i] = m[i] * gr;
         }
-      memcpy(gprev, nprev_g, N * sizeof(double));
-      memcpy(gcur, ncur_g, N * sizeof(double));
-      for (int a = 0; a < 3; ++a)
-        if (F[a]) memcpy(gpsi[a], ngpsi[a], F[a] * sizeof(double));
-      INJECT(n, gcur);
+        /* lap = K cur + sum_a 0.5/h diff(npsi + psi) */
+        or_laplacian(P, lapbar, tmpc2);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < N; ++i) ncur_g[i] += tmpc2[i];
+        if (pml)
+          for (int a = 0; a < 3; ++a) {
+            if (!F[a]) continue;
+            /* sbar = diff^T(lapbar) * 0.5/h feeds both npsi and psi */
+            diff_faces_T(P, a, lapbar, tmpf);
+#pragma omp parallel for schedule(static)
+            for (long f = 0; f < F[a]; ++f) {
+              tmpf[f] *= 0.5 / P->h[a];
+              /* total adjoint of npsi: from later steps (gpsi) + this term */
+              tmpf2[f] = gpsi[a][f] + tmpf[f];
+              /* npsi = alpha psi + beta fd(cur) */
+              ngpsi[a][f] = tmpf[f] + P->alpha[a][f] * tmpf2[f];
+              tmpf[f] = P->beta[a][f] * tmpf2[f];
+            }
+            face_diff_T(P, a, tmpf, tmpc);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < N; ++i) ncur_g[i] += tmpc[i];
+          }
+        double* t;
+        t = gprev; gprev = nprev_g; nprev_g = t;
+        t = gcur; gcur = ncur_g; ncur_g = t;
+        for (int a = 0; a < 3; ++a) {
+          t = gpsi[a]; gpsi[a] = ngpsi[a]; ngpsi[a] = t;
+        }
+        INJECT(n, gcur);
+      }
     }
 #undef INJECT
     if (grad_prev0) memcpy(grad_prev0 + (long)s * N, gprev, N * sizeof(double));
     if (grad_p0) memcpy(grad_p0 + (long)s * N, gcur, N * sizeof(double));
   }
+#undef SLOT_PSI
   free(hist);
+  free(snap);
+  free(laps);
   free(work);
   return 0;
+}
+
+/* Store-all form: every forward state kept, no recomputation. */
+int or_gradient(const or_problem* P, const double* m, const double* p_prev0, const double* p0,
+                const double* residual, double* grad_m, double* grad_prev0, double* grad_p0) {
+  return or_gradient_ckpt(P, m, p_prev0, p0, residual, grad_m, grad_prev0, grad_p0,
+                          P->n_steps, NULL);
 }

@@ -9,6 +9,8 @@ central finite differences, brute-force transposes on tiny grids and textbook sp
 from __future__ import annotations
 
 import math
+import os
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -183,3 +185,56 @@ def test_oracle_order2_1d_courant_one_is_an_exact_shift():
     inner = slice(5, n - 5)
     assert np.abs(cur[inner] - want[inner]).max() < 1e-12
     assert L.Problem is Problem
+
+
+@pytest.mark.parametrize("absorbing", ["none", "sponge", "cerjan", "pml"])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_oracle_checkpointed_gradient_is_bit_identical_to_store_all(absorbing, d):
+    """Snapshots every K steps with per-block recomputation (K dividing the step count or not,
+    K = 1, K = N) give exactly the bits of the store-all reverse sweep."""
+    if d == 3 and absorbing == "pml":
+        pytest.skip("the reference refuses a 3-D PML")
+    P = _tiny(absorbing, order=2 if d == 3 else 4, d=d)
+    rng = np.random.default_rng(5)
+    c = 1.8 + 0.3 * rng.random(P.shape)
+    w = rng.standard_normal((2, 3, P.n_t))
+    g_all = P.fwi_gradient(c, w)
+    for every in (1, 4, 5, P.n_steps - 1, P.n_steps, 10 * P.n_steps):
+        assert np.array_equal(P.fwi_gradient(c, w, every=every), g_all), every
+
+
+@pytest.mark.parametrize("name", ["grad_sponge", "pml2d", "cerjan2d"])
+def test_oracle_checkpointed_gradient_matches_reference_autograd(name):
+    meta, z = load(name)
+    P = oracle_problem(meta, z)
+    every = max(1, P.n_steps // 3 - 1)
+    if meta["grad"] == "weights":
+        g = P.fwi_gradient(z["in_c"], z["in_weights"], every=every)
+    else:
+        g = P.fwi_gradient(z["in_c"], P.fwi_traces(z["in_c"]) - z["f64_d_obs"], every=every)
+    assert rel(g, z["f64_grad"]) < 1e-12
+
+
+def test_oracle_results_do_not_depend_on_the_thread_count():
+    """The OpenMP loops write disjoint cells and sum nothing across them: one thread and many
+    threads give the same bits (PML, the layer with the most parallel loops)."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys, numpy as np; sys.path[:0] = [sys.argv[1], sys.argv[1] + '/tests'];"
+        "import test_oracle as t;"
+        "P = t._tiny('pml', d=2); rng = np.random.default_rng(2);"
+        "c = 1.8 + 0.3 * rng.random(P.shape); w = rng.standard_normal((2, 3, P.n_t));"
+        "g = P.fwi_gradient(c, w, every=4); y = P.fwi_traces(c);"
+        "sys.stdout.write(np.concatenate([g.ravel(), y.ravel()]).tobytes().hex())"
+    )
+    root = str(Path(__file__).resolve().parent.parent)
+    outs = []
+    for threads in ("1", "7"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads)
+        r = subprocess.run([sys.executable, "-c", code, root], env=env, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout)
+    assert outs[0] == outs[1] and len(outs[0]) > 0
