@@ -54,6 +54,21 @@ int32_t nw_filter_apply(nw_filter* f, const void* x, const void* x_sub, const vo
                         void* y, int64_t rows, int32_t filter, double scale, double* sums,
                         void* stream);
 
+/* Allocation-free form of nw_filter_apply. It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc), which create their plans beforehand. Arguments,
+ * results and bits are those of nw_filter_apply; the float64 partial sums of the workgroups go
+ * to a fixed array of NW_FILTER_NOALLOC_PARTIALS doubles (32 KiB) that the library holds on each
+ * GPU it runs on until the process ends, instead of the plan's own buffer. The call allocates
+ * nothing. Every call on one GPU shares that array and leaves the last call's partial sums in
+ * it (each call writes every partial it reads, so nothing needs clearing): calls that overlap
+ * on one GPU (other streams or threads) race, and the caller must order them. A call that
+ * would need more partial sums than the array holds is refused with NW_ERR_ARG. */
+#define NW_FILTER_NOALLOC_PARTIALS 4096
+int32_t nw_filter_apply_noalloc(nw_filter* f, const void* x, const void* x_sub,
+                                const void* mask, void* y, int64_t rows, int32_t filter,
+                                double scale, double* sums, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,7 @@ FILTER_EXPORTS = (
     "nw_filter_destroy",
     "nw_filter_length",
     "nw_filter_apply",
+    "nw_filter_apply_noalloc",
 )
 # every symbol include/nefi_stepper.h declares
 STEPPER_EXPORTS = (
@@ -164,6 +165,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_filter_destroy": (i32, [vp]),
         "nw_filter_length": (i64, [vp]),
         "nw_filter_apply": (i32, [vp, vp, vp, vp, vp, i64, i32, c_double, vp, vp]),
+        "nw_filter_apply_noalloc": (i32, [vp, vp, vp, vp, vp, i64, i32, c_double, vp, vp]),
         "nw_ts_laplacian": (i32, [POINTER(NwTsGrid), vp, vp, vp]),
         "nw_ts_wave_step": (i32, [POINTER(NwTsGrid), c_double, vp, vp, vp, vp, vp, vp, i32, vp]),
         "nw_ts_mur": (i32, [POINTER(NwTsGrid), i32, c_double, vp, vp, vp, vp]),

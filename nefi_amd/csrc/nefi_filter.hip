@@ -20,6 +20,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,8 @@ namespace {
 
 constexpr int kFilterThreads = 256;
 constexpr int kFilterBlocks = 2048;  // workgroups of one apply (a loop over row pairs)
+static_assert(2 * kFilterBlocks <= NW_FILTER_NOALLOC_PARTIALS,
+              "the allocation-free apply's array must hold every workgroup's partial sums");
 constexpr int kMinLog2 = 6;
 // FFT lengths: 16384 float2 (+1/8 padding: 144 KiB of LDS) or 8192 double2 (144 KiB)
 constexpr double kPi = 3.141592653589793238462643383279502884;
@@ -385,9 +388,46 @@ void dispatch_fft(int log2L, const FilterIO<T>& io, int blocks, hipStream_t st) 
   }
 }
 
+// Partial sums of the allocation-free apply: one array per GPU, part of the code object, held
+// until the process ends. Host code reaches it only through the per-device address that
+// hipGetSymbolAddress returns (scratch_on_current_device), never through its name.
+static __device__ __align__(256) double scratch[NW_FILTER_NOALLOC_PARTIALS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(NW_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(NW_ERR_UNSUPPORTED, "nw_filter_apply_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(NW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NW_OK;
+}
+
+// Workgroups one apply launches (each writes two partial sums).
+int apply_blocks(const nw_filter* f, int64_t rows, int filter) {
+  if (rows <= 0) return 0;
+  if (filter) return (int)std::min<int64_t>((rows + 1) / 2, kFilterBlocks);
+  const int64_t chunks = (rows * f->n + 4 * kFilterThreads - 1) / (4 * kFilterThreads);
+  return (int)std::min<int64_t>(chunks, kFilterBlocks);
+}
+
+// `partial`: 2 doubles per workgroup for the sums (the plan's buffer or the per-GPU array)
 template <typename T>
 int apply_t(nw_filter* f, const void* x, const void* xs, const void* mask, void* y, int64_t rows,
-            int filter, double scale, double* sums, hipStream_t st) {
+            int filter, double scale, double* sums, double* partial, hipStream_t st) {
   FilterIO<T> io;
   io.x = (const T*)x;
   io.xs = (const T*)xs;
@@ -398,24 +438,20 @@ int apply_t(nw_filter* f, const void* x, const void* xs, const void* mask, void*
   io.rows = rows;
   io.n = f->n;
   io.scale = (T)scale;
-  io.partial = sums ? f->partial : nullptr;
-  int blocks = 0;
+  io.partial = sums ? partial : nullptr;
+  const int blocks = apply_blocks(f, rows, filter);
   if (rows > 0) {
     const int pid = prof_begin(NW_K_FILTER, st, rows);
-    if (filter) {
-      blocks = (int)std::min<int64_t>((rows + 1) / 2, kFilterBlocks);
+    if (filter)
       dispatch_fft<T>(f->log2L, io, blocks, st);
-    } else {
-      const int64_t chunks = (rows * f->n + 4 * kFilterThreads - 1) / (4 * kFilterThreads);
-      blocks = (int)std::min<int64_t>(chunks, kFilterBlocks);
+    else
       hipLaunchKernelGGL((k_filter_ident<T>), dim3(blocks), dim3(kFilterThreads), 0, st, io);
-    }
     prof_end(pid, st);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(NW_ERR_HIP, std::string("k_filter: ") + hipGetErrorString(e));
   }
   if (sums) {  // (no partials when rows == 0: the sums are zero)
-    hipLaunchKernelGGL(k_filter_sums, dim3(1), dim3(kFilterThreads), 0, st, f->partial, blocks,
+    hipLaunchKernelGGL(k_filter_sums, dim3(1), dim3(kFilterThreads), 0, st, partial, blocks,
                        sums);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -484,16 +520,43 @@ int32_t nw_filter_destroy(nw_filter* f) {
 
 int64_t nw_filter_length(const nw_filter* f) { return f ? f->L : 0; }
 
+static int check_apply(const char* who, const nw_filter* f, const void* x, int64_t rows) {
+  if (!f || (!x && rows > 0)) return fail(NW_ERR_ARG, std::string(who) + ": null argument");
+  if (rows < 0) return fail(NW_ERR_ARG, std::string(who) + ": rows must be >= 0");
+  if (rows * f->n >= ((int64_t)1 << 40)) return fail(NW_ERR_ARG, std::string(who) + ": too many samples");
+  return NW_OK;
+}
+
 int32_t nw_filter_apply(nw_filter* f, const void* x, const void* x_sub, const void* mask,
                         void* y, int64_t rows, int32_t filter, double scale, double* sums,
                         void* stream) {
-  if (!f || (!x && rows > 0)) return fail(NW_ERR_ARG, "nw_filter_apply: null argument");
-  if (rows < 0) return fail(NW_ERR_ARG, "nw_filter_apply: rows must be >= 0");
-  if (rows * f->n >= ((int64_t)1 << 40)) return fail(NW_ERR_ARG, "nw_filter_apply: too many samples");
+  const int rc = check_apply("nw_filter_apply", f, x, rows);
+  if (rc != NW_OK) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (f->dtype == NW_F32)
-    return nw::apply_t<float>(f, x, x_sub, mask, y, rows, filter, scale, sums, st);
-  return nw::apply_t<double>(f, x, x_sub, mask, y, rows, filter, scale, sums, st);
+    return nw::apply_t<float>(f, x, x_sub, mask, y, rows, filter, scale, sums, f->partial, st);
+  return nw::apply_t<double>(f, x, x_sub, mask, y, rows, filter, scale, sums, f->partial, st);
+}
+
+int32_t nw_filter_apply_noalloc(nw_filter* f, const void* x, const void* x_sub,
+                                const void* mask, void* y, int64_t rows, int32_t filter,
+                                double scale, double* sums, void* stream) {
+  int rc = check_apply("nw_filter_apply_noalloc", f, x, rows);
+  if (rc != NW_OK) return rc;
+  const int blocks = nw::apply_blocks(f, rows, filter);
+  if (2 * (int64_t)blocks > NW_FILTER_NOALLOC_PARTIALS)
+    return fail(NW_ERR_ARG, "nw_filter_apply_noalloc: " + std::to_string(blocks) +
+                                " workgroups need more partial sums than the " +
+                                std::to_string(NW_FILTER_NOALLOC_PARTIALS) + " the array holds");
+  double* partial = nullptr;
+  if (sums) {
+    rc = nw::scratch_on_current_device(&partial);
+    if (rc != NW_OK) return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (f->dtype == NW_F32)
+    return nw::apply_t<float>(f, x, x_sub, mask, y, rows, filter, scale, sums, partial, st);
+  return nw::apply_t<double>(f, x, x_sub, mask, y, rows, filter, scale, sums, partial, st);
 }
 
 }  // extern "C"
