@@ -21,8 +21,9 @@ launch advances several steps) over its average duration, timed with HIP events 
 launch. Temporal blocking moves fewer bytes than that algorithmic count, so for the blocked
 kernel ``achieved`` is an effective rate and ``traffic`` (measured HBM bytes) shows the real
 one.
-``cpu_baseline`` times the FP64 oracle (same physics, C, OpenMP) on a bounded sample of the
-same workload on this machine's host cores.
+``cpu_baseline`` times the reference's CPU path (its op sequence in PyTorch on the CPU, float32,
+all host threads; one thread and the FP64 oracle beside it) on a bounded sample of the same
+workload on this machine's host cores.
 """
 
 from __future__ import annotations
@@ -74,34 +75,84 @@ def survey(n=1024, shots=64, n_t=4001, seed=0, absorbing="sponge"):
     return dict(n=n, dx=dx, L=L, c=c, shots=shot_geo, kw=kw, counts=counts)
 
 
-def cpu_baseline(wl, budget_s=15.0):
-    """FP64 oracle gradient on shot 0 of the same grid, step count scaled to ~budget_s."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(wl, op, budget_s=10.0):
+    """The reference's CPU path on this machine's host cores: the reference's own op sequence
+    (``reference_leapfrog``: conv2d Laplacian, damping products, dense source map, gather,
+    torch.utils.checkpoint every 200 steps, autograd backward) in float32 on the CPU, for shot 0
+    of the workload's grid, over a bounded number of steps scaled to ~``budget_s`` seconds,
+    with all host threads and with one thread. The FP64 oracle (C, OpenMP) on the same sample
+    is a second entry."""
+    import torch
+
     from oracle.leapfrog import Problem
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = wl["n"]
+    cells = op.grid.padded_shape[0] * op.grid.padded_shape[1]
+    nproc = os.cpu_count() or 1
+    threads_all = int(os.environ.get("OMP_NUM_THREADS", nproc))
+    c = torch.tensor(wl["c"], dtype=torch.float32)
+    rows = op.receiver_counts[0]
+    res = torch.randn(rows, op.n_t, generator=torch.Generator().manual_seed(1))
+    saved_threads = torch.get_num_threads()
+
+    def run_ref(threads):
+        torch.set_num_threads(threads)
+        reference_leapfrog(op, c, [0], 2, res, block=200)  # warm-up
+        t0 = time.perf_counter()
+        reference_leapfrog(op, c, [0], 10, res, block=200)
+        t = time.perf_counter() - t0
+        steps = int(max(10, min(op.n_steps, budget_s / max(t, 1e-3) * 10)))
+        t0 = time.perf_counter()
+        reference_leapfrog(op, c, [0], steps, res, block=200)
+        t = time.perf_counter() - t0
+        return {"value": round(cells * steps / t / 1e9, 5), "unit": "Gcell-updates/s",
+                "cores": threads, "kind": "port",
+                "sample": f"the reference's op sequence (conv2d leapfrog, checkpoint every 200 "
+                          f"steps, autograd backward) in float32 on the CPU with {threads} "
+                          f"thread(s): 1 shot x {steps} steps on the {cells}-cell padded grid "
+                          f"in {t:.1f} s, gradient (forward + recompute + backward)"}
+
+    try:
+        main = run_ref(threads_all)
+        one = run_ref(1)
+    finally:
+        torch.set_num_threads(saved_threads)
+
     src, rec = wl["shots"][0]
     kw = dict(wl["kw"])
 
-    def run(n_t):
+    def run_oracle(n_t):
         kw["n_t"] = n_t
         P = Problem((n, n), ((0.0, wl["L"]), (0.0, wl["L"])), rec, sources=src, **kw)
-        res = np.random.default_rng(1).standard_normal((1, len(rec), n_t))
+        r = np.random.default_rng(1).standard_normal((1, len(rec), n_t))
         t0 = time.perf_counter()
-        P.fwi_gradient(wl["c"], res)
-        return time.perf_counter() - t0, P.cells * P.n_steps
+        P.fwi_gradient(wl["c"], r, every=200)
+        return time.perf_counter() - t0, P.cells * P.n_steps, P.n_steps
 
-    t, work = run(6)
-    steps = int(max(5, min(400, (budget_s / max(t, 1e-3)) * 5)))
-    t, work = run(steps + 1)
-    return {
-        "value": work / t / 1e9,
-        "unit": "Gcell-updates/s",
-        "cores": threads,
+    t, work, _ = run_oracle(6)
+    steps = int(max(5, min(4000, (budget_s / max(t, 1e-3)) * 5)))
+    t, work, steps = run_oracle(steps + 1)
+    main["one_thread"] = one
+    main["oracle_fp64"] = {
+        "value": round(work / t / 1e9, 5), "unit": "Gcell-updates/s", "cores": threads_all,
         "kind": "port",
-        "sample": f"FP64 oracle gradient (forward storing every state + reverse sweep), 1 shot, "
-                  f"{steps} steps on the 1064^2 padded grid, {t:.1f} s",
-    }
+        "sample": f"FP64 oracle gradient (C, OpenMP; snapshots every 200 steps, block "
+                  f"recomputation), 1 shot x {steps} steps on the same grid, {t:.1f} s"}
+    main["cpu_model"] = _cpu_model()
+    main["nproc"] = nproc
+    return main
 
 
 def reference_leapfrog(op, c, shot_ids, n_steps, residual=None, block=200):
@@ -194,15 +245,21 @@ def autograd_leapfrog(op, c, shots, n_steps, residual, block=200):
     return reference_leapfrog(op, c, range(shots), n_steps, residual[:rows], block)[1]
 
 
-def autograd_comparator(op, c, residual, steps=200):
-    """Time the plain autograd path with every step's tape kept (grad_mode="autograd") and
-    with checkpoint blocks of 200 (grad_mode="checkpoint"), each at the source batch that fits
-    HBM for the whole workload, and scale linearly to the workload (one batch of the
-    store-all path's tape over all 4000 steps would not fit, so `steps` steps are timed).
+def autograd_comparator(op, c, residual, repeats=3, ckpt_steps=1000):
+    """Time the plain PyTorch autograd path on the same GPU: with every step's tape kept
+    (grad_mode="autograd") and with checkpoint blocks of 200 (grad_mode="checkpoint"), each at
+    the source batch that fits HBM for the whole workload.
+
+    Store-all: one batch of that size over all of the workload's steps, scaled only in shots.
+    Checkpointed: every shot in one batch (it fits), over ``ckpt_steps`` steps, scaled in
+    steps (its cost per step is the same in every block). Each is run ``repeats`` times after
+    a warm-up; the min and the median are reported, the median is used.
 
     Batch sizing: the tape bytes per shot-step are measured on a short probe; store-all keeps
     n_steps of them per shot, checkpointing keeps one block of 200 plus two wavefields per
     snapshot. The budget is 85 % of the free device memory."""
+    import statistics
+
     import torch
 
     n_shots = len(op.shot_sources)
@@ -222,49 +279,71 @@ def autograd_comparator(op, c, residual, steps=200):
     out = {}
     for mode, block in (("autograd", None), ("checkpoint", 200)):
         batch = int(max(1, min(n_shots, budget // per_shot[mode])))
+        steps = op.n_steps if mode == "autograd" else min(op.n_steps, ckpt_steps)
         autograd_leapfrog(op, c, batch, 20, residual, block=block)  # warm up this batch
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        autograd_leapfrog(op, c, batch, steps, residual, block=block)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            autograd_leapfrog(op, c, batch, steps, residual, block=block)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] autograd comparator {mode}: {batch} shots x {steps} steps "
+                  f"{times[-1]:.2f} s", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
         scale = (op.n_steps / steps) * (n_shots / batch)
+        med = statistics.median(times)
         out[mode] = {
-            "ms_per_gradient_scaled": round(t * scale * 1e3, 1),
+            "ms_per_gradient_scaled": round(med * scale * 1e3, 1),
+            "ms_per_gradient_scaled_min": round(min(times) * scale * 1e3, 1),
             "source_batch": batch,
+            "repeats": repeats,
             "sample": f"source_batch {batch} (tape {tape / 2**20:.1f} MiB per shot-step, "
                       f"{per_shot[mode] / 2**30:.1f} GiB per shot, budget "
                       f"{budget / 2**30:.0f} GiB): {batch} shots x {steps} steps measured "
-                      f"{t:.2f} s, scaled x{scale:.0f} to {n_shots} shots x {op.n_steps} steps",
+                      f"{repeats} times, median {med:.2f} s, min {min(times):.2f} s, scaled "
+                      f"x{scale:g} to {n_shots} shots x {op.n_steps} steps",
         }
         torch.cuda.empty_cache()
     return out
 
 
-def gradient_error_vs_fp64(wl, shots=8):
-    """Relative L2 error of the float32 gradient against the float64 one (the float64 HIP
-    path, equal to the reference's autograd to ~1e-15 in the parity tests) at the workload's
-    grid and time length, on the first `shots` shots."""
+def gradient_error_vs_fp64(op, wl, c32):
+    """Accuracy of the timed plan itself: the production operator ``op`` (64 shots, the timed
+    plan, its graphs and kept blocks) differentiates a residual that is nonzero on two shots
+    (shot 0 and the shot with the most receivers); its float32 traces and gradient are compared
+    with the float64 HIP path on those two shots (which equals the FP64 oracle to ~1e-14 at this
+    size and the reference's float64 autograd to ~1e-15 in the parity tests)."""
     import torch
 
-    from nefi_amd.domain import Domain
     from nefi_amd.physics.batched import ShotBatchWaveOperator
 
-    n = wl["n"]
-    dom = Domain((n, n), ((0.0, wl["L"]), (0.0, wl["L"])))
-    op = ShotBatchWaveOperator(dom, wl["shots"][:shots], grad_mode="checkpoint",
-                               checkpoint_every=200, **wl["kw"])
-    rows = int(sum(wl["counts"][:shots]))
+    counts = [int(k) for k in wl["counts"]]
+    picked = [0, int(np.argmax(counts))]
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    rows = np.concatenate([np.arange(starts[s], starts[s + 1]) for s in picked])
     gen = torch.Generator("cuda").manual_seed(7)
-    res64 = torch.randn(rows, op.n_t, device="cuda", dtype=torch.float64, generator=gen)
-    out = {}
-    for dt in (torch.float32, torch.float64):
-        x = torch.tensor(wl["c"], dtype=dt, device="cuda", requires_grad=True)
-        y = op({"c": x})
-        (g,) = torch.autograd.grad((y * res64.to(dt)).sum(), x)
-        out[dt] = g.double()
-    err = float((out[torch.float32] - out[torch.float64]).norm() / out[torch.float64].norm())
-    return {"rel_l2": err, "sample": f"{shots} shots, full grid and {op.n_steps} steps"}
+    res64 = torch.zeros(int(starts[-1]), op.n_t, device="cuda", dtype=torch.float64)
+    rsel = torch.as_tensor(rows, device="cuda")
+    res64[rsel] = torch.randn(len(rows), op.n_t, device="cuda", dtype=torch.float64,
+                              generator=gen)
+    x = c32.detach().clone().requires_grad_(True)
+    y = op({"c": x})
+    (g32,) = torch.autograd.grad((y * res64.float()).sum(), x)
+    y32 = y.detach()[rsel].double()
+    sub = ShotBatchWaveOperator(op.domain, [wl["shots"][s] for s in picked],
+                                grad_mode="checkpoint", checkpoint_every=200, **wl["kw"])
+    x64 = c32.detach().double().requires_grad_(True)
+    y64 = sub({"c": x64})
+    (g64,) = torch.autograd.grad((y64 * res64[rsel]).sum(), x64)
+    eg = float((g32.double() - g64).norm() / g64.norm())
+    et = float((y32 - y64.detach()).norm() / y64.detach().norm())
+    del sub
+    torch.cuda.empty_cache()
+    return {"rel_l2": eg, "traces_rel_l2": et,
+            "sample": f"the timed {len(op.shot_sources)}-shot plan with the residual on shots "
+                      f"{picked}, full grid and {op.n_steps} steps, against the float64 HIP "
+                      f"path on those shots"}
 
 
 def misfit_bench(rows, n_t, dt_obs, f_max, dev, reps=5):
@@ -728,7 +807,10 @@ def main() -> int:
     if not args.no_misfit:
         misfit = misfit_bench(rows, op.n_t, op.dt_obs, 0.5 * wl["kw"]["f0"], dev)
         print(f"[bench] misfit: {misfit}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
+    grad_err = None
+    if not args.no_accuracy:
+        grad_err = gradient_error_vs_fp64(op, wl, c)
+        print(f"[bench] accuracy of the timed plan: {grad_err}", file=sys.stderr, flush=True)
     comp = None
     if not args.no_autograd:
         op._cache.clear()  # release the HIP path's workspace: the comparator gets the device
@@ -736,8 +818,12 @@ def main() -> int:
         comp = autograd_comparator(op, c, residual)
         for entry in comp.values():
             entry["speedup"] = round(entry["ms_per_gradient_scaled"] / ms_per_step, 1)
+            entry["speedup_vs_min"] = round(entry["ms_per_gradient_scaled_min"] / ms_per_step, 1)
         print(f"[bench] autograd comparator: {comp}", file=sys.stderr, flush=True)
-    grad_err = None if args.no_accuracy else gradient_error_vs_fp64(wl)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, op)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
     line = {
         "metric": "fwd+adjoint Gcell-updates/s (64-shot 2-D FWI gradient)",
         "value": round(value, 3),
