@@ -451,6 +451,11 @@ constexpr int kLW = 256 + 8;  // LDS row: 4 margin columns on each side, 16-byte
 // LDS row of a split tile (two half-width tiles side by side, each with its 4 margin columns)
 constexpr int kLW2 = 2 * (128 + 8);
 
+// Workgroup barrier that orders LDS only: waits for this wavefront's LDS accesses, not for its
+// outstanding global loads and stores (which __syncthreads' release fence would drain). The
+// memory clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 template <int TY, int R>
 struct TileShape {
   static constexpr int rows = TY + 2 * R;
@@ -516,12 +521,14 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
   const float4 hx = (ABS != NW_NONE && act) ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<ABS> dcol = damp4_col<ABS>(hx);
   const float w_n = io.src_taps > 0 ? io.wavelet[io.n] : 0.f;
-  // model rows of this wavefront: shot-independent, loaded once for the group
+  // model rows and damping terms of this wavefront: shot-independent, loaded once for the group
   float4 mm[S::my_rows];
+  float hyv[S::my_rows];
 #pragma unroll
   for (int k = 0; k < S::my_rows; ++k) {
     const int y = y0 + r0 + k;
     mm[k] = (act && y < D1) ? ld4(io.m + (y * P2 + x)) : f4(0.f);
+    hyv[k] = (ABS != NW_NONE && y < D1) ? g.damp[1][y] : 0.f;
   }
   const int s_end = min(shots, (tl.grp + 1) * spw);
   for (int s = tl.grp * spw; s < s_end; ++s) {
@@ -549,8 +556,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
         const float4 nb = lnb(lp, r0 + k + R, lane);
         const int ro = y * P2 + x;
         const float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
-        const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
+        const Damp4<ABS> dm = damp4_row<ABS>(hyv[k], hx, dcol);
         const float4 pc = w[R];
         const int tpv = io.taper_prev;
         float4 o = make_float4(rhs1<ABS>(pc.x, pp[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
@@ -581,9 +587,33 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
   }
 }
 
+// Waves of one k3_adjoint workgroup: 8, one tile row each at 8-row tiles. At 8 rows a wave's
+// registers stay below 128, so 4 waves share each SIMD, and every load of a shot (the two tiles
+// with their halos and the wave's own rows) is in flight at once.
+template <int TY>
+struct AdjWaves {
+  static constexpr int waves = TY < 8 ? TY : 8;
+};
+
+template <int TY, int R>
+struct AdjShape {
+  static constexpr int waves = AdjWaves<TY>::waves;
+  static constexpr int threads = 64 * waves;
+  static constexpr int rows = TY + 2 * R;
+  static constexpr int my_rows = TY / waves;
+  static constexpr int n4 = rows * (kLW / 4);  // float4 elements of one tile
+  static constexpr int per_thread = (n4 + threads - 1) / threads;
+  static constexpr int n4s = rows * (kLW2 / 4);  // ... of a split tile
+  static constexpr int per_thread_s = (n4s + threads - 1) / threads;
+};
+
 // One tile of k3_adjoint. Q tiles (tile, halo and margins inside the grid, output cells in the
 // box of neutral damping) skip every bounds check and step with a = b = 1, as k4_tile does.
-// The tile of m a_eff is shot-independent: it stays in registers for the whole shot group.
+// The tile of m a_eff is shot-independent: it stays in registers for the whole shot group, and
+// so do the rows' damping terms. Per shot, every load (the lambda^{n+1} and p^n tiles with
+// their halos, the wave's lambda^{n+1}, lambda^{n+2} rows) is issued before the first barrier,
+// so a shot costs one memory round trip; the barriers order LDS only, so the previous shot's
+// stores need not have completed.
 // SP ("split", never quiet): two half-width tiles of the grid's last 128 or fewer columns, row
 // tiles 2j (lanes 0-31) and 2j + 1 (lanes 32-63). An LDS row (kLW2 floats) holds both tiles'
 // rows, each between its own margin columns, so every lane's x-neighbours are its own tile's.
@@ -593,7 +623,8 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
                                             float* lp) {
   static_assert(!(Q && SP), "split tiles lie on the grid's edge");
   constexpr int ABS = Q ? NW_NONE : ABS0;
-  using S = TileShape<TY, R>;
+  using S = AdjShape<TY, R>;
+  constexpr int NT = S::threads;
   constexpr int LW = SP ? kLW2 : kLW;                             // LDS row pitch
   constexpr int PT = SP ? S::per_thread_s : S::per_thread;        // tile loads per thread
   constexpr int N4 = SP ? S::n4s : S::n4;
@@ -626,7 +657,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   auto load_tile = [&](const float* src, float4* v) {
 #pragma unroll
     for (int k = 0; k < PT; ++k) {
-      const int i = threadIdx.x + 256 * k;
+      const int i = threadIdx.x + NT * k;
       const int row = i / (LW / 4);
       int c4 = i % (LW / 4), ty0 = yA;
       if (SP && c4 >= kLW2 / 8) {  // the lower tile's half of the LDS row
@@ -634,7 +665,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
         ty0 = yA + TY;
       }
       const int gy = ty0 - R + row, gx = x0 - 4 + 4 * c4;
-      const bool in = 256 * (k + 1) <= N4 || i < N4;
+      const bool in = NT * (k + 1) <= N4 || i < N4;
       const bool ok = in && (Q || (gy >= 0 && gy < D1 && gx >= 0 && gx < D2));
       v[k] = ok ? ld4(src + (gy * P2 + gx)) : f4(0.f);
     }
@@ -642,12 +673,19 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   auto store_tile = [&](float* lds, const float4* v) {
 #pragma unroll
     for (int k = 0; k < PT; ++k) {
-      const int i = threadIdx.x + 256 * k;
+      const int i = threadIdx.x + NT * k;
       if (i < N4) *reinterpret_cast<float4*>(lds + 4 * i) = v[k];
     }
   };
   float4 vm[PT];
   load_tile(io.ma, vm);
+  // damping terms of the wave's rows (a scalar per row, loaded once, not per shot)
+  float hyv[S::my_rows];
+#pragma unroll
+  for (int k = 0; k < S::my_rows; ++k) {
+    const int y = y0 + r0 + k;
+    hyv[k] = (ABS != NW_NONE && act && y < D1) ? g.damp[1][y] : 0.f;
+  }
   // float64 imaging sums of this wavefront's cells over the shot group
   double2 ga[S::my_rows], gb[S::my_rows];
 #pragma unroll
@@ -659,29 +697,24 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   for (int s = tl.grp * spw; s < s_end; ++s) {
     const int64_t base = (int64_t)s * g.cells;
     const float* L1 = io.lam1 + base;
-    {
-      float4 vl[PT];
-      load_tile(L1, vl);
-      __syncthreads();  // the previous shot's reads of lq / lp are done
-#pragma unroll
-      for (int k = 0; k < PT; ++k) vl[k] = mul4(vm[k], vl[k]);
-      store_tile(lq, vl);
-    }
-    if (p_tile) {
-      float4 vp[PT];
-      load_tile(io.p_n + base, vp);
-      store_tile(lp, vp);
-    }
+    float4 vl[PT], vp[PT];
+    load_tile(L1, vl);
+    if (p_tile) load_tile(io.p_n + base, vp);
     float4 lc[S::my_rows], l2c[S::my_rows], lpn[S::my_rows];
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
       const bool ok = Q || (act && y < D1);
-      lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // just loaded into the tile: cache hit
+      lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // also in the tile: a cache hit
       l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
       lpn[k] = (ok && imaging && kept_lap) ? ld4(io.lap_n + base + (y * P2 + x)) : f4(0.f);
     }
-    __syncthreads();
+    lds_barrier();  // the previous shot's reads of lq / lp are done
+#pragma unroll
+    for (int k = 0; k < PT; ++k) vl[k] = mul4(vm[k], vl[k]);
+    store_tile(lq, vl);
+    if (p_tile) store_tile(lp, vp);
+    lds_barrier();
     float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
@@ -696,8 +729,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
       if (Q || (act && y < D1)) {
         const float4 qn = nb4(lq, r0 + k + R);
         const float4 lapq = stencil4<R>(g, qw, qn.x, qn.y, qn.z, qn.w);
-        const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
-        const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
+        const Damp4<ABS> dm = damp4_row<ABS>(hyv[k], hx, dcol);
         float4 rb;
         const float4 o = make_float4(
             adj1<ABS>(lc[k].x, l2c[k].x, lapq.x, dm.a.x, dm.b.x, dm.tap.x, tpn, rb.x),
@@ -744,9 +776,11 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
 }
 
 template <int R, int ABS, int TY>
-__global__ void __launch_bounds__(256) k3_adjoint(Geo<float> g, AdjIO<float> io, int shots,
-                                                  int spw, int xcd_map) {
-  using S = TileShape<TY, R>;
+__global__ void __launch_bounds__(64 * AdjWaves<TY>::waves) k3_adjoint(Geo<float> g,
+                                                                      AdjIO<float> io,
+                                                                      int shots, int spw,
+                                                                      int xcd_map) {
+  using S = AdjShape<TY, R>;
   __shared__ __align__(16) float lq[S::rows * kLW2];  // q = (m a_eff) lambda^{n+1}
   __shared__ __align__(16) float lp[S::rows * kLW2];  // p^n
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
@@ -811,11 +845,6 @@ struct BlockShape {
 
 typedef __attribute__((address_space(1))) void gmem_void;
 typedef __attribute__((address_space(3))) void lds_void;
-
-// Workgroup barrier that orders LDS only: waits for this wavefront's LDS accesses, not for its
-// outstanding global-to-LDS copies (which __syncthreads' release fence would drain). The
-// memory clobber keeps the compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Issue the loads of one region of a quiet tile (every element inside the grid) straight into
 // LDS rows R.. of `buf` (global_load_lds: lane l of a wavefront lands 16 bytes after lane l-1,
@@ -2187,8 +2216,9 @@ static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots,
     const int tiles = block_tiles(adj_cols((int)g.D[2]), div_up((int)g.D[1], lds_rows<TY>()),
                                   div_up(shots, spw));
     hipLaunchKernelGGL((k3_adjoint<R, ABS, lds_rows<TY>()>),
-                       dim3((unsigned)(kXcds * div_up(tiles, kXcds))), dim3(256), 0, st, g, io,
-                       shots, spw, xm & 1);
+                       dim3((unsigned)(kXcds * div_up(tiles, kXcds))),
+                       dim3(64 * AdjWaves<lds_rows<TY>()>::waves), 0, st, g, io, shots, spw,
+                       xm & 1);
   } else {
     hipLaunchKernelGGL((k2_adjoint<R, ABS, TY>), dim3(grid_tiles(g, TY, shots, spw)), dim3(256),
                        0, st, g, io, shots, spw, xm & 1);
