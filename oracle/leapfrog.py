@@ -68,6 +68,8 @@ def lib():
         h.or_gradient.argtypes = [POINTER(_Prob), vp, vp, vp, vp, vp, vp, vp]
         h.or_gradient_ckpt.argtypes = [POINTER(_Prob), vp, vp, vp, vp, vp, vp, vp, c_int, vp]
         h.or_laplacian.argtypes = [POINTER(_Prob), vp, vp]
+        h.or_forward_clamped.argtypes = [POINTER(_Prob), vp, vp, vp, vp, c_int, vp, vp]
+        h.or_forward_clamped.restype = c_int
         h.or_forward.restype = h.or_gradient.restype = h.or_gradient_ckpt.restype = c_int
         _lib = h
     return _lib
@@ -346,3 +348,40 @@ class Problem:
         assert rc == 0
         g = gcur + gprev + self.laplacian(0.5 * m * gprev)
         return self.crop(g.reshape(B, *self.padded))
+
+    # ---- time reversal (wave.py:878-968) ----------------------------------------------------
+    def reversed_traces(self, traces):
+        """Traces at reversed time ``T - n dt`` for n = 0..n_steps, linearly interpolated
+        between samples (wave.py:941-947): ``(n_steps + 1, n_receivers)``."""
+        tr = np.asarray(traces, np.float64)
+        n_t, m = tr.shape[-1], self.substeps
+        out = np.empty((self.n_steps + 1, tr.shape[0]))
+        for n in range(self.n_steps + 1):
+            u = (n_t - 1) - n / m
+            j0 = min(max(int(math.floor(u)), 0), n_t - 1)
+            j1 = min(j0 + 1, n_t - 1)
+            t = u - j0
+            out[n] = (1.0 - t) * tr[:, j0] + t * tr[:, j1]
+        return out
+
+    def time_reversal(self, traces, c, mode="dirichlet"):
+        """``time_reversal`` of the initial-value operator on this problem's grid: the receiver
+        clamp propagated from rest (``dirichlet``: p^{n} at the receivers' nearest cells set to
+        the reversed traces after every step, wave.py:949-967) or the adjoint image
+        ``A^T traces`` (``adjoint``, wave.py:943-947)."""
+        if mode == "adjoint":
+            return self.ic_gradient(c, np.asarray(traces, np.float64)[None])[0]
+        m = self.model(c)
+        vals = np.ascontiguousarray(self.reversed_traces(traces))
+        cells = np.ascontiguousarray(self.nearest_index(self.receivers), dtype=np.int64)
+        p0 = np.zeros(self.cells)
+        p0[cells] = vals[0]
+        z = np.zeros(self.cells)
+        st = np.zeros((2, self.cells))
+        P = self._struct(1)
+        rc = lib().or_forward_clamped(ctypes.byref(P), m.ctypes.data, z.ctypes.data,
+                                      p0.ctypes.data, cells.ctypes.data, len(cells),
+                                      vals.ctypes.data, st.ctypes.data)
+        assert rc == 0
+        return self.crop(st[1].reshape(self.padded))
+

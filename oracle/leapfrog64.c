@@ -277,6 +277,49 @@ int or_forward(const or_problem* P, const double* m, const double* p_prev0, cons
   return 0;
 }
 
+/* Forward run of one shot from (p_prev0, p0) in which, after every step n -> n+1, the cells
+ * cells[0..n_cells-1] of p^{n+1} are set to values[(n+1) * n_cells + k]: the receiver clamp of
+ * time_reversal's Dirichlet mode (nefi/physics/wave.py:958-962). p0 is taken as given (the
+ * caller clamps it for n = 0). The final state is written to state_out (2, cells). */
+int or_forward_clamped(const or_problem* P, const double* m, const double* p_prev0,
+                       const double* p0, const long* cells, int n_cells, const double* values,
+                       double* state_out) {
+  const long N = cells_of(P);
+  long Fmax = 0;
+  for (int a = 0; a < 3; ++a)
+    if (fcells_of(P, a) > Fmax) Fmax = fcells_of(P, a);
+  double* buf = (double*)calloc((size_t)(4 * N + 6 * Fmax + Fmax + N + N), sizeof(double));
+  if (!buf) return 1;
+  double *pa = buf, *pb = buf + N, *qa = buf + 2 * N, *qb = buf + 3 * N;
+  double* psi_a[3];
+  double* psi_b[3];
+  double* fb = buf + 4 * N;
+  for (int a = 0; a < 3; ++a) {
+    psi_a[a] = fb + 2 * a * Fmax;
+    psi_b[a] = fb + (2 * a + 1) * Fmax;
+  }
+  double* tmpf = buf + 4 * N + 6 * Fmax;
+  double* lap = tmpf + Fmax;
+  double* tmpc = lap + N;
+  memcpy(pa, p_prev0, N * sizeof(double));
+  memcpy(pb, p0, N * sizeof(double));
+  double *prev = pa, *cur = pb, *nprev = qa, *ncur = qb;
+  double** psi = psi_a;
+  double** npsi = psi_b;
+  for (int n = 0; n < P->n_steps; ++n) {
+    step(P, m, 0, n, prev, cur, psi, nprev, ncur, npsi, lap, tmpf, tmpc);
+    for (int k = 0; k < n_cells; ++k) ncur[cells[k]] = values[(long)(n + 1) * n_cells + k];
+    double* t;
+    t = prev; prev = nprev; nprev = t;
+    t = cur; cur = ncur; ncur = t;
+    double** u = psi; psi = npsi; npsi = u;
+  }
+  memcpy(state_out, prev, N * sizeof(double));
+  memcpy(state_out + N, cur, N * sizeof(double));
+  free(buf);
+  return 0;
+}
+
 /* Gradient of J = sum(residual * traces) with respect to m (summed over shots, (cells)) and
  * to the initial state of every shot (grad_prev0, grad_p0: (S, cells); either may be NULL).
  *

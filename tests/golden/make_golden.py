@@ -25,7 +25,12 @@ sys.path.insert(0, os.environ.get("NEFI_REFERENCE", "/root/reference"))
 
 import cases  # noqa: E402
 from nefi.domain import Domain  # noqa: E402
-from nefi.physics.wave import WaveInitialConditionOperator, WaveOperator  # noqa: E402
+from nefi.physics.wave import (  # noqa: E402
+    WaveInitialConditionOperator,
+    WaveOperator,
+    laplacian,
+    time_reversal,
+)
 
 
 def _op(case, dtype):
@@ -63,8 +68,60 @@ def run(case, dtype):
     return out
 
 
+def time_reversal_golden():
+    """``time_reversal`` images of the reference in both modes, float64 and float32, from the
+    traces the reference itself computed for the pat2d and pat3d cases (their f64 traces)."""
+    blob, meta = {}, {}
+    for make in (cases.pat2d, cases.pat3d):
+        case = make()
+        name = case["name"]
+        z = np.load(HERE / f"{name}.npz", allow_pickle=False)
+        traces = z["f64_traces"][0] if z["f64_traces"].ndim == 3 else z["f64_traces"]
+        kw = {k: v for k, v in case["kw"].items() if k not in ("n_t",)}
+        c = kw.pop("c", 1.0)
+        blob[f"{name}_in_traces"] = traces
+        meta[name] = {"shape": case["shape"], "extent": case["extent"],
+                      "receivers": case["receivers"], "kw": kw,
+                      "c_map": isinstance(c, np.ndarray)}
+        if isinstance(c, np.ndarray):
+            blob[f"{name}_in_c"] = c
+        for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+            cc = torch.as_tensor(c, dtype=dtype) if isinstance(c, np.ndarray) else float(c)
+            for mode in ("dirichlet", "adjoint"):
+                img = time_reversal(torch.as_tensor(traces, dtype=dtype),
+                                    Domain(case["shape"], case["extent"]), case["receivers"],
+                                    c=cc, mode=mode, **kw)
+                blob[f"{name}_{mode}_{tag}"] = img.numpy()
+    blob["meta"] = np.array(json.dumps(meta, default=lambda o: o.tolist()))
+    np.savez_compressed(HERE / "time_reversal.npz", **blob)
+    print("time_reversal", {k: getattr(v, "shape", None) for k, v in blob.items()}, flush=True)
+
+
+def laplacian_golden():
+    """The reference's ``laplacian`` at orders 2 and 4 on seeded batches of 1-D, 2-D and 3-D
+    grids with unequal spacings, float64 and float32."""
+    rng = np.random.default_rng(21)
+    blob, meta = {}, {}
+    for d, shape, spacing in ((1, (3, 17), (0.3,)), (2, (2, 9, 13), (0.5, 0.7)),
+                              (3, (2, 6, 7, 5), (0.4, 0.6, 0.9))):
+        p = rng.standard_normal(shape)
+        blob[f"d{d}_in"] = p
+        meta[f"d{d}"] = {"spacing": spacing}
+        for order in (2, 4):
+            for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+                out = laplacian(torch.as_tensor(p, dtype=dtype), spacing, order=order)
+                blob[f"d{d}_o{order}_{tag}"] = out.numpy()
+    blob["meta"] = np.array(json.dumps(meta))
+    np.savez_compressed(HERE / "laplacian.npz", **blob)
+    print("laplacian", {k: getattr(v, "shape", None) for k, v in blob.items()}, flush=True)
+
+
 def main(names=None):
     torch.set_num_threads(max(1, min(8, os.cpu_count() or 1)))
+    if not names or "time_reversal" in names:
+        time_reversal_golden()
+    if not names or "laplacian" in names:
+        laplacian_golden()
     for make in cases.ALL:
         case = make()
         if names and case["name"] not in names:

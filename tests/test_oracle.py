@@ -238,3 +238,47 @@ def test_oracle_results_do_not_depend_on_the_thread_count():
         assert r.returncode == 0, r.stderr
         outs.append(r.stdout)
     assert outs[0] == outs[1] and len(outs[0]) > 0
+
+
+def _tr_case(name):
+    import json
+
+    from _support import GOLDEN
+
+    z = np.load(GOLDEN / "time_reversal.npz", allow_pickle=False)
+    meta = json.loads(str(z["meta"]))[name]
+    traces = z[f"{name}_in_traces"]
+    c = z[f"{name}_in_c"] if meta["c_map"] else np.full(tuple(meta["shape"]), 1.0)
+    P = Problem(meta["shape"], meta["extent"], meta["receivers"], n_t=traces.shape[-1],
+                c_max=float(np.max(c)), **meta["kw"])
+    return z, P, traces, c
+
+
+@pytest.mark.parametrize("name", ["pat2d", "pat3d"])
+@pytest.mark.parametrize("mode", ["dirichlet", "adjoint"])
+def test_oracle_time_reversal_matches_reference(name, mode):
+    """The receiver clamp propagated from rest (Dirichlet mode) and the adjoint image, against
+    the reference's own float64 time_reversal images (PML ring in 2-D, sponge in 3-D)."""
+    z, P, traces, c = _tr_case(name)
+    img = P.time_reversal(traces, c, mode)
+    assert img.shape == tuple(z[f"{name}_{mode}_f64"].shape)
+    assert rel(img, z[f"{name}_{mode}_f64"]) < 1e-12
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("order", [2, 4])
+def test_oracle_laplacian_matches_reference(d, order):
+    import json
+
+    from _support import GOLDEN
+
+    z = np.load(GOLDEN / "laplacian.npz", allow_pickle=False)
+    spacing = json.loads(str(z["meta"]))[f"d{d}"]["spacing"]
+    x = z[f"d{d}_in"]
+    shape = x.shape[1:]
+    P = Problem(shape, [(0.0, n * h) for n, h in zip(shape, spacing)],
+                [[0.5 * n * h for n, h in zip(shape, spacing)]], n_t=2, dt_obs=1e-3,
+                c_max=1.0, order=order, absorbing="none")
+    assert P.padded == tuple(shape)
+    out = P.laplacian(x.reshape(x.shape[0], -1)).reshape(x.shape)
+    assert rel(out, z[f"d{d}_o{order}_f64"]) < 1e-13
