@@ -1,14 +1,15 @@
-"""Rectangular physical domain on a uniform cell-centred grid (``nefi/domain.py:19-156``).
+"""The box a field lives on and its uniform grid of cells (the contract of ``nefi/domain.py``).
 
-The wave path needs the spacing, the resolution change ``at`` and the coordinate grids; their
-conventions (cell centres at ``lo + (i + ½)·h``, normalised coordinates in ``[-1, 1]``) are the
-reference's, because receiver and source stencils are derived from them.
+Only what the wave operators read is provided: the grid ``shape``, the box ``extent``, the cell
+size per axis, the same box at another resolution, and the cell-centre coordinates. The
+conventions are nefi's because receiver and source stencils are computed from them: cell ``i``
+of an axis spanning ``[lo, hi)`` with ``n`` cells is centred at ``lo + (i + 1/2)(hi - lo)/n``,
+and normalised coordinates map the box onto ``[-1, 1]``.
 """
 
 from __future__ import annotations
 
 from collections.abc import Sequence
-from dataclasses import dataclass, field
 
 import torch
 
@@ -16,116 +17,96 @@ from .errors import ShapeError
 from .utils import shape_tuple
 
 
-@dataclass(frozen=True)
 class Domain:
-    """Grid ``shape`` over the box ``extent = ((lo, hi), ...)``; optional axis names."""
+    """``shape`` cells over the box ``extent = ((lo, hi), ...)``, optionally named ``axes``.
 
-    shape: tuple[int, ...]
-    extent: tuple[tuple[float, float], ...]
-    axes: tuple[str, ...] | None = field(default=None)
+    Immutable and hashable: two domains with the same grid and box are equal.
+    """
 
-    def __post_init__(self) -> None:
-        shape = shape_tuple(self.shape)
-        ext = tuple((float(lo), float(hi)) for lo, hi in self.extent)
-        object.__setattr__(self, "shape", shape)
-        object.__setattr__(self, "extent", ext)
-        if len(ext) != len(shape):
-            raise ShapeError(f"extent has {len(ext)} axes but shape {shape} has {len(shape)}")
-        if any(not hi > lo for lo, hi in ext):
-            raise ShapeError(f"extent must satisfy hi > lo, got {ext}")
-        if self.axes is not None:
-            names = tuple(self.axes)
-            if len(names) != len(shape):
-                raise ShapeError("axes must have one name per dimension")
-            object.__setattr__(self, "axes", names)
+    __slots__ = ("_shape", "_extent", "_axes")
 
-    @staticmethod
-    def unit(shape, axes: Sequence[str] | None = None) -> Domain:
-        shape = shape_tuple(shape)
-        return Domain(shape, tuple((0.0, 1.0) for _ in shape), tuple(axes) if axes else None)
+    def __init__(self, shape, extent, axes: Sequence[str] | None = None) -> None:
+        grid = shape_tuple(shape)
+        box = tuple((float(a), float(b)) for a, b in extent)
+        if len(box) != len(grid):
+            raise ShapeError(f"extent has {len(box)} axes but shape {grid} has {len(grid)}")
+        bad = [ax for ax, (a, b) in enumerate(box) if not b > a]
+        if bad:
+            raise ShapeError(f"extent must satisfy hi > lo on every axis, not on axes {bad}")
+        names = None if axes is None else tuple(str(n) for n in axes)
+        if names is not None and len(names) != len(grid):
+            raise ShapeError(f"{len(names)} axis names for a {len(grid)}-D domain")
+        object.__setattr__(self, "_shape", grid)
+        object.__setattr__(self, "_extent", box)
+        object.__setattr__(self, "_axes", names)
 
-    @staticmethod
-    def from_spacing(shape, spacing, origin=0.0, axes: Sequence[str] | None = None) -> Domain:
-        shape = shape_tuple(shape)
-        d = len(shape)
-        sp = tuple(spacing) if isinstance(spacing, (tuple, list)) else (float(spacing),) * d
-        og = tuple(origin) if isinstance(origin, (tuple, list)) else (float(origin),) * d
-        ext = tuple((float(o), float(o + n * s)) for n, s, o in zip(shape, sp, og))
-        return Domain(shape, ext, tuple(axes) if axes else None)
+    def __setattr__(self, name, value):
+        raise AttributeError("Domain is immutable")
+
+    @classmethod
+    def unit(cls, shape, axes: Sequence[str] | None = None) -> Domain:
+        """``shape`` cells over the unit box ``[0, 1]^d``."""
+        grid = shape_tuple(shape)
+        return cls(grid, [(0.0, 1.0)] * len(grid), axes)
+
+    # ---- the grid and the box ------------------------------------------------------------
+    @property
+    def shape(self) -> tuple[int, ...]:
+        return self._shape
+
+    @property
+    def extent(self) -> tuple[tuple[float, float], ...]:
+        return self._extent
+
+    @property
+    def axes(self) -> tuple[str, ...] | None:
+        return self._axes
 
     @property
     def ndim(self) -> int:
-        return len(self.shape)
+        return len(self._shape)
 
-    @property
-    def size(self) -> tuple[float, ...]:
-        return tuple(hi - lo for lo, hi in self.extent)
-
-    @property
-    def numel(self) -> int:
-        out = 1
-        for n in self.shape:
-            out *= n
-        return out
+    def _grid(self, shape) -> tuple[int, ...]:
+        grid = self._shape if shape is None else shape_tuple(shape)
+        if len(grid) != self.ndim:
+            raise ShapeError(f"a {len(grid)}-D shape {grid} on a {self.ndim}-D domain")
+        return grid
 
     def spacing(self, shape=None) -> tuple[float, ...]:
-        """Cell size per axis at ``shape`` (default: the native shape)."""
-        shape = self.shape if shape is None else shape_tuple(shape)
-        if len(shape) != self.ndim:
-            raise ShapeError(f"shape {shape} does not match domain ndim {self.ndim}")
-        return tuple((hi - lo) / n for (lo, hi), n in zip(self.extent, shape))
+        """Cell size per axis on a grid of ``shape`` cells (default: this domain's grid)."""
+        grid = self._grid(shape)
+        return tuple((b - a) / n for (a, b), n in zip(self._extent, grid))
 
     def at(self, shape) -> Domain:
-        """The same box at another resolution."""
-        shape = shape_tuple(shape)
-        if len(shape) != self.ndim:
-            raise ShapeError(f"shape {shape} does not match domain ndim {self.ndim}")
-        return Domain(shape, self.extent, self.axes)
+        """The same box on a grid of ``shape`` cells."""
+        return Domain(self._grid(shape), self._extent, self._axes)
 
-    def coarsen(self, factor: int = 2, min_size: int = 4) -> Domain:
-        return self.at(tuple(max(min_size, n // factor) for n in self.shape))
-
-    def refine(self, factor: int = 2) -> Domain:
-        return self.at(tuple(n * factor for n in self.shape))
-
-    def axis_coords(self, shape=None, normalized: bool = True, device=None, dtype=None):
-        shape = self.shape if shape is None else shape_tuple(shape)
+    # ---- coordinates of the cell centres -------------------------------------------------
+    def _centres(self, shape, normalized: bool, device, dtype) -> list[torch.Tensor]:
         dtype = dtype or torch.get_default_dtype()
         out = []
-        for (lo, hi), n in zip(self.extent, shape):
-            i = torch.arange(n, device=device, dtype=dtype)
-            out.append(-1.0 + (2.0 * i + 1.0) / n if normalized else lo + (i + 0.5) * (hi - lo) / n)
+        for (a, b), n in zip(self._extent, self._grid(shape)):
+            half = torch.arange(n, device=device, dtype=dtype) + 0.5  # i + 1/2
+            out.append(2.0 * half / n - 1.0 if normalized else a + half * (b - a) / n)
         return out
 
     def coords(self, shape=None, device=None, dtype=None) -> torch.Tensor:
-        """Normalised cell centres, ``(*shape, ndim)`` in ``[-1, 1]``."""
-        mesh = torch.meshgrid(*self.axis_coords(shape, True, device, dtype), indexing="ij")
-        return torch.stack(mesh, dim=-1)
+        """Normalised cell centres in ``[-1, 1]``, ``(*shape, ndim)``."""
+        axes = self._centres(shape, True, device, dtype)
+        return torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1)
 
     def physical_coords(self, shape=None, device=None, dtype=None) -> torch.Tensor:
         """Physical cell centres, ``(*shape, ndim)``."""
-        mesh = torch.meshgrid(*self.axis_coords(shape, False, device, dtype), indexing="ij")
-        return torch.stack(mesh, dim=-1)
+        axes = self._centres(shape, False, device, dtype)
+        return torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1)
 
-    def to_normalized(self, x: torch.Tensor) -> torch.Tensor:
-        lo = torch.tensor([e[0] for e in self.extent], device=x.device, dtype=x.dtype)
-        hi = torch.tensor([e[1] for e in self.extent], device=x.device, dtype=x.dtype)
-        return 2.0 * (x - lo) / (hi - lo) - 1.0
+    # ---- value semantics -------------------------------------------------------------------
+    def __eq__(self, other) -> bool:
+        return isinstance(other, Domain) and (self._shape, self._extent, self._axes) == (
+            other._shape, other._extent, other._axes)
 
-    def to_physical(self, u: torch.Tensor) -> torch.Tensor:
-        lo = torch.tensor([e[0] for e in self.extent], device=u.device, dtype=u.dtype)
-        hi = torch.tensor([e[1] for e in self.extent], device=u.device, dtype=u.dtype)
-        return lo + (u + 1.0) * 0.5 * (hi - lo)
+    def __hash__(self) -> int:
+        return hash((self._shape, self._extent, self._axes))
 
-    def to_dict(self) -> dict:
-        return {
-            "shape": list(self.shape),
-            "extent": [list(e) for e in self.extent],
-            "axes": list(self.axes) if self.axes else None,
-        }
-
-    @staticmethod
-    def from_dict(d: dict) -> Domain:
-        axes = d.get("axes")
-        return Domain(tuple(d["shape"]), tuple(tuple(e) for e in d["extent"]),
-                      tuple(axes) if axes else None)
+    def __repr__(self) -> str:
+        return f"Domain(shape={self._shape}, extent={self._extent}, axes={self._axes})"

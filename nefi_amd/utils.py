@@ -16,31 +16,33 @@ def shape_tuple(shape) -> tuple[int, ...]:
     return tuple(int(n) for n in shape)
 
 
-def resample(x: torch.Tensor, shape: Sequence[int], mode: str = "auto") -> torch.Tensor:
-    """Resample the trailing ``len(shape)`` dims to ``shape``.
+_LINEAR = {1: "linear", 2: "bilinear", 3: "trilinear"}
 
-    ``auto`` averages over areas when every axis shrinks and interpolates (bi/tri)linearly with
-    cell-centred alignment otherwise, as ``nefi/utils/tensor.py:35-69`` does. The wave operators
-    call it on the host, once, for the known sound speed of the photoacoustic operator.
+
+def resample(x: torch.Tensor, shape: Sequence[int], mode: str = "auto") -> torch.Tensor:
+    """``x`` with its trailing ``len(shape)`` dims resampled to ``shape``.
+
+    The rule of ``nefi/utils/tensor.py:35-69``: ``auto`` takes cell averages (``area``) when no
+    axis grows and cell-centred (bi/tri)linear interpolation otherwise; ``area`` and ``nearest``
+    can be asked for directly. Used on the host, once, for the known sound speed of the
+    photoacoustic operator at another resolution.
     """
-    shape = shape_tuple(shape)
-    d = len(shape)
-    if d not in (1, 2, 3):
-        raise ShapeError(f"resample supports 1-3 spatial dims, got target shape {shape}")
+    target = shape_tuple(shape)
+    d = len(target)
+    if d not in _LINEAR:
+        raise ShapeError(f"resample supports 1-3 spatial dims, got target shape {target}")
     if x.ndim < d:
-        raise ShapeError(f"tensor with shape {tuple(x.shape)} has fewer dims than target {shape}")
-    old = tuple(x.shape[-d:])
-    if old == shape:
+        raise ShapeError(f"tensor with shape {tuple(x.shape)} has fewer dims than target {target}")
+    source = tuple(x.shape[-d:])
+    if source == target:
         return x
-    lead = tuple(x.shape[:-d])
-    xb = x.reshape(-1, 1, *old)
     if mode == "auto":
-        mode = "area" if all(o >= n for o, n in zip(old, shape)) else "linear"
+        mode = "area" if all(s >= t for s, t in zip(source, target)) else "linear"
     if mode == "linear":
-        kind = ("linear", "bilinear", "trilinear")[d - 1]
-        y = F.interpolate(xb, size=shape, mode=kind, align_corners=False)
+        how = dict(mode=_LINEAR[d], align_corners=False)
     elif mode in ("area", "nearest"):
-        y = F.interpolate(xb, size=shape, mode=mode)
+        how = dict(mode=mode)
     else:
         raise ValueError(f"unknown resample mode {mode!r}")
-    return y.reshape(*lead, *shape)
+    out = F.interpolate(x.reshape((-1, 1) + source), size=target, **how)
+    return out.reshape(tuple(x.shape[:-d]) + target)

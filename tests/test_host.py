@@ -303,3 +303,102 @@ def test_kept_blocks_fall_back_to_the_one_stream_sweep_when_two_buffers_do_not_f
     assert kept(20, held=("keep", "keep2")) == 2  # the held pair is reused
     assert kept(20, held=("keep",)) == 1
     assert kept(30, held=("keep", "keep2", "keep3", "keep4")) == 4
+
+
+def test_errors_are_nefis_own_classes_when_nefi_is_importable(tmp_path):
+    """With nefi importable (the plugin case) the wave path raises nefi's classes, so nefi's
+    ``except NefiError`` (cli.py) catches them: checked in a subprocess against a stand-in
+    ``nefi.errors`` module with nefi's class names. Standalone, the local hierarchy is used."""
+    import subprocess
+    import sys
+
+    pkg = tmp_path / "nefi"
+    pkg.mkdir()
+    (pkg / "__init__.py").write_text("")
+    (pkg / "errors.py").write_text(
+        "class NefiError(Exception): pass\n"
+        "class ShapeError(NefiError, ValueError): pass\n"
+        "class ConfigError(NefiError, ValueError): pass\n"
+        "class RegistryError(NefiError, KeyError): pass\n"
+        "class SolverError(NefiError, RuntimeError): pass\n"
+        "class OperatorError(NefiError, RuntimeError): pass\n")
+    code = (
+        "import sys, torch; sys.path[:0] = [sys.argv[1], sys.argv[2]]\n"
+        "import nefi.errors as ne, nefi_amd.errors as e\n"
+        "from nefi_amd.domain import Domain\n"
+        "from nefi_amd.physics.wave import WaveOperator\n"
+        "assert e.BOUND_TO_NEFI and e.OperatorError is ne.OperatorError\n"
+        "assert e.ConfigError is ne.ConfigError and issubclass(e.NativeError, ne.NefiError)\n"
+        "op = WaveOperator(Domain((8, 8), ((0., 1.), (0., 1.))), [[.5, .5]], [[.2, .2]],"
+        " n_t=4, dt_obs=0.01, f0=5.0, c_max=2.0, absorbing='sponge', absorb_width=0.25)\n"
+        "try:\n"
+        "    op({'c': torch.full((8, 8), 3.0)})\n"
+        "except ne.NefiError as err:\n"
+        "    assert 'exceeds' in str(err); print('caught', type(err).__name__)\n"
+    )
+    root = str(Path(__file__).resolve().parent.parent)
+    r = subprocess.run([sys.executable, "-c", code, str(tmp_path), root], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "caught OperatorError" in r.stdout
+    import nefi_amd.errors as local
+
+    assert not local.BOUND_TO_NEFI and issubclass(local.OperatorError, local.NefiError)
+
+
+def test_domain_contract():
+    from nefi_amd.domain import Domain
+    from nefi_amd.errors import ShapeError
+
+    d = Domain((4, 5), ((0.0, 2.0), (-1.0, 1.5)))
+    assert d.ndim == 2 and d.shape == (4, 5) and d.spacing() == (0.5, 0.5)
+    assert d.spacing((8, 10)) == (0.25, 0.25)
+    assert d.at((8, 10)) == Domain((8, 10), d.extent) and hash(d.at((4, 5))) == hash(d)
+    x = d.physical_coords(dtype=torch.float64)
+    assert x.shape == (4, 5, 2)
+    assert torch.allclose(x[:, 0, 0], torch.tensor([0.25, 0.75, 1.25, 1.75], dtype=torch.float64))
+    assert torch.allclose(x[0, :, 1], torch.tensor([-0.75, -0.25, 0.25, 0.75, 1.25],
+                                                   dtype=torch.float64))
+    u = d.coords(dtype=torch.float64)
+    assert torch.allclose(u[:, 0, 0], torch.tensor([-0.75, -0.25, 0.25, 0.75], dtype=torch.float64))
+    assert Domain.unit((8,)).extent == ((0.0, 1.0),)
+    with pytest.raises(ShapeError):
+        Domain((4,), ((1.0, 0.0),))
+    with pytest.raises(ShapeError):
+        Domain((4, 4), ((0.0, 1.0),))
+    with pytest.raises(ShapeError):
+        d.at((4,))
+    with pytest.raises(AttributeError):
+        d.shape = (1, 1)
+
+
+def test_sensor_bandwidth_attributes_and_resolution_copies():
+    from nefi_amd.instances.photoacoustic3d import SensorBandwidth
+    from nefi_amd.operators.base import Operator
+
+    class Inner(Operator):
+        primary, homogeneity = "p0", 1.0
+
+        def __init__(self, n):
+            super().__init__()
+            self.n = n
+            self.fidelity_tag = "wave-leapfrog-o2-sponge"
+
+        def at_resolution(self, shape):
+            return self if tuple(shape) == (self.n,) else Inner(shape[0])
+
+        def output_shape(self, shape):
+            return (3, 7)
+
+    op = SensorBandwidth(Inner(16), 0.01, 12.5)
+    assert op.fidelity_tag == "wave-leapfrog-o2-sponge-lp12.5"
+    assert op.primary == "p0" and op.homogeneity == 1.0 and op.required_fields() == ("p0",)
+    assert op.output_shape((16,)) == (3, 7) and not op.traceable
+    assert op.at_resolution((16,)) is op
+    half = op.at_resolution((8,))
+    assert half is op.at_resolution([8]) and half.inner.n == 8
+    assert half.fidelity_tag == op.fidelity_tag and half.f_cutoff == 12.5
+    ideal = SensorBandwidth(Inner(16), 0.01, None)
+    assert ideal.fidelity_tag == "wave-leapfrog-o2-sponge" and ideal.f_cutoff is None
+    x = torch.ones(2, 5)
+    assert ideal._filter(x) is x
