@@ -153,8 +153,10 @@ int32_t nw_model(nw_plan* plan, const void* c, void* m, void* ma, void* stream);
  *   n % ckpt_every == 0 the state (p^{n-1}, p^n[, psi]) is written to checkpoint slot
  *   n / ckpt_every (slot size = nw_ckpt_slot_bytes).
  * keep / keep_begin: if keep is not NULL, p^{n} for n in [keep_begin, n_end] are also written
- *   to keep slot n - keep_begin (slot = one wavefield buffer; PML: followed by the psi fields),
- *   which the reverse sweep reads instead of recomputing them.
+ *   to keep slot n - keep_begin (slot = one wavefield buffer; PML: followed by the psi fields,
+ *   except when the slots hold Laplacians, NW_OPT_KEEP_LAP), which the reverse sweep reads
+ *   instead of recomputing them. The slot size (nw_keep_slot_bytes) depends on the plan's
+ *   tiles and options: query it after changing them.
  * first_step: the step index at which the simulation started (Cerjan tapers p^{n-1} only for
  *   n > first_step). */
 int32_t nw_forward(nw_plan* plan, const void* m, void** state, void** psi, int32_t n_begin,
@@ -236,7 +238,8 @@ int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);
  * nw_grad_slices * padded cells doubles; nw_model_grad sums all of them). */
 int64_t nw_grad_slices(const nw_plan* plan);
 
-/* Bytes of one checkpoint slot and of one keep slot. */
+/* Bytes of one checkpoint slot and of one keep slot (the latter for the current tiles and
+ * options: slots of kept Laplacians hold one wavefield buffer, PML face fields included in it). */
 int64_t nw_ckpt_slot_bytes(const nw_plan* plan);
 int64_t nw_keep_slot_bytes(const nw_plan* plan);
 

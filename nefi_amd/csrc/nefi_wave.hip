@@ -1582,6 +1582,8 @@ int64_t nw_ckpt_slot_bytes(const nw_plan* P) {
 }
 
 int64_t nw_keep_slot_bytes(const nw_plan* P) {
+  // a slot of kept Laplacians holds no face fields (the reverse step reads none)
+  if (nw::keep_lap(P)) return P->shot_bytes;
   return P->shot_bytes + P->face_bytes[0] + P->face_bytes[1] + P->face_bytes[2];
 }
 

@@ -36,6 +36,22 @@ __device__ __forceinline__ void st4(float* p, const float4& v) {
 }
 __device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
 
+// Read-only table value at a wave-uniform index (profiles, source taps): through the constant
+// address space the compiler may use a scalar load, which waits only on the scalar cache, not
+// on the wave's outstanding vector memory operations.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+  typedef const __attribute__((address_space(4))) T* cptr;
+  return ((cptr)p)[i];
+}
+
+// Index of this lane's wavefront in its workgroup as a wave-uniform value: rows addressed
+// through it are the same for every lane, so their per-row profile values (damping, PML
+// coefficients) are read with scalar loads rather than vector loads that wait on memory.
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // row y of a grid (zero outside the grid or for inactive lanes)
 // offsets within one shot's grid fit in 32 bits (nw_plan_create rejects >= 2^31 cells), which
 // lets loads use a uniform base address plus a 32-bit per-lane offset
@@ -278,9 +294,9 @@ __device__ __forceinline__ void step_row(const Geo<float>& g, const StepIO<float
                          rhs1<ABS>(pc.z, pp.z, mm.z, lap.z, dm.b.z, dm.tap.z, tpv),
                          rhs1<ABS>(pc.w, pp.w, mm.w, lap.w, dm.b.w, dm.tap.w, tpv));
   for (int t = 0; t < io.src_taps; ++t) {
-    const int d = io.src_idx[s * io.src_taps + t] - ro;
+    const int d = ldc(io.src_idx, s * io.src_taps + t) - ro;
     if (d >= 0 && d < 4) {
-      const float v = io.src_val[s * io.src_taps + t] * w_n;
+      const float v = ldc(io.src_val, s * io.src_taps + t) * w_n;
       o.x += d == 0 ? v : 0.f;
       o.y += d == 1 ? v : 0.f;
       o.z += d == 2 ? v : 0.f;
@@ -302,7 +318,7 @@ __global__ void __launch_bounds__(256) k2_step(Geo<float> g, StepIO<float> io, i
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const Tile tl = tile_of(div_up(D2, 256), div_up(D1, 4 * TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x = (tl.cx * 64 + lane) * 4;
   const int ys = (tl.ry * 4 + wave) * TY;
   if (ys >= D1) return;
@@ -365,7 +381,7 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const Tile tl = tile_of(div_up(D2, 256), div_up(D1, 4 * TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x = (tl.cx * 64 + lane) * 4;
   const int ys = (tl.ry * 4 + wave) * TY;
   if (ys >= D1) return;
@@ -513,7 +529,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const Tile tl = tile_of(div_up(D2, 256), div_up(D1, TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;  // first tile row of this wavefront
@@ -564,9 +580,9 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
                                rhs1<ABS>(pc.z, pp[k].z, mm[k].z, lap.z, dm.b.z, dm.tap.z, tpv),
                                rhs1<ABS>(pc.w, pp[k].w, mm[k].w, lap.w, dm.b.w, dm.tap.w, tpv));
         for (int t = 0; t < io.src_taps; ++t) {
-          const int d = io.src_idx[s * io.src_taps + t] - ro;
+          const int d = ldc(io.src_idx, s * io.src_taps + t) - ro;
           if (d >= 0 && d < 4) {
-            const float sv = io.src_val[s * io.src_taps + t] * w_n;
+            const float sv = ldc(io.src_val, s * io.src_taps + t) * w_n;
             o.x += d == 0 ? sv : 0.f;
             o.y += d == 1 ? sv : 0.f;
             o.z += d == 2 ? sv : 0.f;
@@ -629,7 +645,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
   constexpr int PT = SP ? S::per_thread_s : S::per_thread;        // tile loads per thread
   constexpr int N4 = SP ? S::n4s : S::n4;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int hl = SP ? (lane & 31) : lane;  // lane within its tile
   const int x0 = tl.cx * 256, yA = tl.ry * TY;
   const int y0 = SP && lane >= 32 ? yA + TY : yA;
@@ -884,7 +900,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = BlockShape<TY, R, T>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int hl = SP ? (lane & 31) : lane;    // lane within its tile
   const int ryA = tl.ry * TY - S::hy;        // grid row of region row 0 (upper tile if SP)
   const int ry0 = SP && lane >= 32 ? ryA + TY : ryA;  // ... of this lane's tile
@@ -962,7 +978,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     // does a source tap of this shot fall in this wavefront's rows of the region? (uniform)
     bool src_rows = false;
     for (int t = 0; t < io.src_taps; ++t) {
-      const int idx = io.src_idx[s * io.src_taps + t];
+      const int idx = ldc(io.src_idx, s * io.src_taps + t);
       const int sy = idx / P2 - ry0 - c0;
       src_rows |= sy >= 0 && sy < S::mr;
     }
@@ -1035,7 +1051,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           const int c = c0 + k;
           if (lap_out && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1))
             st4(lap_out + base + (y * P2 + x), lap);
-          const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
+          const float hy = ABS != NW_NONE ? (SP ? g.damp[1][y] : ldc(g.damp[1], y)) : 0.f;
           const Damp4<ABS> dm = damp4_row<ABS>(hy, hx, dcol);
           const float4 pc = cur[k];
           o = make_float4(rhs1<ABS>(pc.x, pv[k].x, mm[k].x, lap.x, dm.b.x, dm.tap.x, tpv),
@@ -1045,9 +1061,9 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           if (src) {
             const int ro = y * P2 + x;
             for (int t = 0; t < io.src_taps; ++t) {
-              const int d = io.src_idx[s * io.src_taps + t] - ro;
+              const int d = ldc(io.src_idx, s * io.src_taps + t) - ro;
               if (d >= 0 && d < 4) {
-                const float sv = io.src_val[s * io.src_taps + t] * w_n;
+                const float sv = ldc(io.src_val, s * io.src_taps + t) * w_n;
                 o.x += d == 0 ? sv : 0.f;
                 o.y += d == 1 ? sv : 0.f;
                 o.z += d == 2 ? sv : 0.f;
@@ -1149,7 +1165,7 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
   using S = BlockShape<TY, R, T, kAdjWaves>;
   constexpr int kInjChunks = kTapChunks * 256 / S::threads;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int ry0 = tl.ry * TY - S::hy;
   const int rx0 = tl.cx * S::out_w - S::hx;
   const int x = rx0 + 4 * lane;
@@ -1439,7 +1455,7 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
   using S = TileShape<TY, R>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int F2 = D2 + 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;
@@ -1486,13 +1502,13 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
         const float4 nb = lnb(lp, r0 + k + R, lane);
         float4 lap = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
         const float4 pc = w[R], pm = w[R - 1], pq = w[R + 1];
-        const float zcy = Q ? 0.f : g.zc[1][y];
+        const float zcy = Q ? 0.f : ldc(g.zc[1], y);
         if constexpr (!Q) {
           const bool lq = col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1];
           // y faces of rows y (own) and y + 1
-          const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
-          const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
-          const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+          const float ay0 = ldc(g.alpha[1], y), ay1 = ldc(g.alpha[1], y + 1);
+          const float ky0 = ldc(g.kappa[1], y), ky1 = ldc(g.kappa[1], y + 1);
+          const float zy0 = ldc(g.zf[1], y), zy1 = ldc(g.zf[1], y + 1);
           const float4 oy0 = lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
           const float4 oy1 =
               lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
@@ -1544,7 +1560,7 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
         if constexpr (Q) {
           dm.a = dm.b = dm.tap = f4(1.f);
         } else {
-          dm = damp4_row<NW_PML>(g.damp[1][y], hx, dcol);
+          dm = damp4_row<NW_PML>(ldc(g.damp[1], y), hx, dcol);
         }
         const int ro = y * P2 + x;
         const int tpv = io.taper_prev;
@@ -1561,9 +1577,9 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
           o.w = o.w - (zzr * zcx.w) * pc.w;
         }
         for (int t = 0; t < io.src_taps; ++t) {
-          const int d = io.src_idx[s * io.src_taps + t] - ro;
+          const int d = ldc(io.src_idx, s * io.src_taps + t) - ro;
           if (d >= 0 && d < 4) {
-            const float sv = io.src_val[s * io.src_taps + t] * w_n;
+            const float sv = ldc(io.src_val, s * io.src_taps + t) * w_n;
             o.x += d == 0 ? sv : 0.f;
             o.y += d == 1 ? sv : 0.f;
             o.z += d == 2 ? sv : 0.f;
@@ -1610,7 +1626,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   using S = TileShape<TY, R>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int F2 = D2 + 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;
@@ -1671,10 +1687,10 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
         if constexpr (Q) {
           dm.a = dm.b = dm.tap = f4(1.f);
         } else {
-          dm = damp4_row<NW_PML>(g.damp[1][y], hx, dcol);
+          dm = damp4_row<NW_PML>(ldc(g.damp[1], y), hx, dcol);
         }
         const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
-        const float zcy = Q ? 0.f : g.zc[1][y];
+        const float zcy = Q ? 0.f : ldc(g.zc[1], y);
         const float zzr = g.dt2 * zcy;
         float4 rb, lam;
 #define NW_PML_ADJ(c)                                                                        \
@@ -1697,9 +1713,9 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
           // y faces of rows y and y + 1
           {
             const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
-            const float ay0 = g.alpha[1][y], ay1 = g.alpha[1][y + 1];
-            const float ky0 = g.kappa[1][y], ky1 = g.kappa[1][y + 1];
-            const float zy0 = g.zf[1][y], zy1 = g.zf[1][y + 1];
+            const float ay0 = ldc(g.alpha[1], y), ay1 = ldc(g.alpha[1], y + 1);
+            const float ky0 = ldc(g.kappa[1], y), ky1 = ldc(g.kappa[1], y + 1);
+            const float zy0 = ldc(g.zf[1], y), zy1 = ldc(g.zf[1], y + 1);
             const float4 i0 =
                 lq ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
             const float4 i1 =
@@ -2027,7 +2043,7 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
   const int64_t plane = (int64_t)D1 * P2;
   const Tile3 tl = tile3_of(div_up(D2, 256), div_up(D1, TY), div_up(D0, ZC), shots, xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY, z0 = tl.cz * ZC, s = tl.s;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;
@@ -2057,9 +2073,9 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
                              rhs1<ABS>(pc.z, pp.z, mm.z, lap.z, dm.b.z, dm.tap.z, tpv),
                              rhs1<ABS>(pc.w, pp.w, mm.w, lap.w, dm.b.w, dm.tap.w, tpv));
       for (int t = 0; t < io.src_taps; ++t) {
-        const int64_t d = io.src_idx[s * io.src_taps + t] - ro;
+        const int64_t d = ldc(io.src_idx, s * io.src_taps + t) - ro;
         if (d >= 0 && d < 4) {
-          const float sv = io.src_val[s * io.src_taps + t] * w_n;
+          const float sv = ldc(io.src_val, s * io.src_taps + t) * w_n;
           o.x += d == 0 ? sv : 0.f;
           o.y += d == 1 ? sv : 0.f;
           o.z += d == 2 ? sv : 0.f;
@@ -2090,7 +2106,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
   const int64_t plane = (int64_t)D1 * P2;
   const Tile3 tl = tile3_of(div_up(D2, 256), div_up(D1, TY), div_up(D0, ZC), shots, xcd_map);
   if (!tl.valid) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY, z0 = tl.cz * ZC, s = tl.s;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;

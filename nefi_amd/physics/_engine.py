@@ -186,7 +186,6 @@ class DevicePlan:
         self.handle = handle
         self.esize = 8 if dtype == torch.float64 else 4
         self.ckpt_slot = lib.nw_ckpt_slot_bytes(handle) // self.esize
-        self.keep_slot = lib.nw_keep_slot_bytes(handle) // self.esize
         S, cells = host.n_shots, host.cells
         self.shot_numel = S * cells
         self.face_numel = []
@@ -201,6 +200,12 @@ class DevicePlan:
         if h is not None and nat._LIB is not None:
             nat._LIB.nw_plan_destroy(h)
             self.handle = None
+
+    @property
+    def keep_slot(self) -> int:
+        """Elements of one keep slot for the plan's current tiles and options (a slot of kept
+        Laplacians carries no PML face fields)."""
+        return int(nat.lib().nw_keep_slot_bytes(self.handle)) // self.esize
 
     # ---- tiles of the fast 2-D kernels -----------------------------------------------------
     def tiles(self, kind: int) -> tuple[int, int]:
