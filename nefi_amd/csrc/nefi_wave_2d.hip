@@ -1445,6 +1445,31 @@ __device__ __forceinline__ PmlX pml_x(const Geo<float>& g, int x, bool act) {
   return r;
 }
 
+// Float4 tile loads of a workgroup of S::threads lanes: rows y0 - R .. y0 + TY + R - 1 and
+// columns x0 - 4 .. x0 + 259 of one grid (zero outside it), element i of the LDS tile per lane
+// i, i + threads, ...
+template <typename S, int R>
+__device__ __forceinline__ void tile_load_w(const float* src, int y0, int x0, int D1, int D2,
+                                            int P2, float4* v) {
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + S::threads * k;
+    const int row = i / (kLW / 4), c4 = i % (kLW / 4);
+    const int gy = y0 - R + row, gx = x0 - 4 + 4 * c4;
+    const bool ok = i < S::n4 && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
+    v[k] = ok ? ld4(src + (gy * P2 + gx)) : f4(0.f);
+  }
+}
+
+template <typename S>
+__device__ __forceinline__ void tile_store_w(float* lds, const float4* v) {
+#pragma unroll
+  for (int k = 0; k < S::per_thread; ++k) {
+    const int i = threadIdx.x + S::threads * k;
+    if (i < S::n4) *reinterpret_cast<float4*>(lds + 4 * i) = v[k];
+  }
+}
+
 // One tile of k3_step_pml. Q: the tile lies in the PML's quiet box, where every face-field
 // coefficient and the damping vanish: it neither reads nor writes face fields (they are zero
 // there and stay zero; face buffers, snapshots and keep slots hold zeros in the box from
@@ -1615,21 +1640,25 @@ __global__ void __launch_bounds__(256) k3_step_pml(Geo<float> g, StepIO<float> i
     k3_step_pml_tile<R, TY, false>(g, io, shots, spw, tl, lp);
 }
 
-// One tile of k3_adjoint_pml. Q: the tile lies in the PML's quiet box, where every face
-// coefficient and the damping vanish: the adjoint face fields there enter lambda (here and in
-// the neighbouring tiles) only multiplied by zero, so the tile neither reads nor writes them
-// (they keep the finite values of the initial state) and steps with a = b = 1.
+// One tile of k3_adjoint_pml (8 wavefronts, AdjShape rows each). Q: the tile lies in the PML's
+// quiet box, where every face coefficient and the damping vanish: the adjoint face fields
+// there enter lambda (here and in the neighbouring tiles) only multiplied by zero, so the tile
+// neither reads nor writes them (they keep the finite values of the initial state) and steps
+// with a = b = 1. The m a_eff tile is shot-independent and stays in registers; per shot every
+// load (the lambda^{n+1} tile, the wave's lambda rows, kept Laplacians and the adjoint face
+// fields its rows read) is issued before the first barrier, which orders LDS only.
 template <int R, int TY, bool Q>
 __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const AdjIO<float>& io,
                                                     int shots, int spw, const Tile& tl,
                                                     float* lq, float* lp) {
-  using S = TileShape<TY, R>;
+  using S = AdjShape<TY, R>;
+  constexpr int MR = S::my_rows;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int F2 = D2 + 1;
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
-  const int r0 = wave * S::my_rows;
+  const int r0 = wave * MR;
   const bool act = x < D2;
   const bool last_col = x + 4 == D2;
   const float4 hx = act ? ld4(g.damp[2] + x) : f4(0.f);
@@ -1642,9 +1671,11 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   const bool p_tile = imaging && !kept_lap;
   // lanes whose four cells lie in the quiet box skip their face loads and stores as well
   const bool col_quiet = x >= g.quiet[2][0] && x + 4 <= g.quiet[2][1];
-  double2 ga[S::my_rows], gb[S::my_rows];
+  float4 vm[S::per_thread];
+  tile_load_w<S, R>(io.ma, y0, x0, D1, D2, P2, vm);
+  double2 ga[MR], gb[MR];
 #pragma unroll
-  for (int k = 0; k < S::my_rows; ++k) {
+  for (int k = 0; k < MR; ++k) {
     ga[k] = make_double2(0.0, 0.0);
     gb[k] = ga[k];
   }
@@ -1653,21 +1684,32 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
     const int64_t base = (int64_t)s * g.cells;
     const int64_t fy = (int64_t)s * g.fcells[1], fx = (int64_t)s * g.fcells[2];
     const float* L1 = io.lam1 + base;
-    {
-      float4 vl[S::per_thread], vm[S::per_thread];
-      tile_load<TY, R>(L1, y0, x0, D1, D2, P2, vl);
-      tile_load<TY, R>(io.ma, y0, x0, D1, D2, P2, vm);
-      __syncthreads();  // the previous shot's reads of lq / lp are done
+    float4 vl[S::per_thread], vp[S::per_thread];
+    tile_load_w<S, R>(L1, y0, x0, D1, D2, P2, vl);
+    if (p_tile) tile_load_w<S, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
+    // this wave's rows: lambda^{n+1}, lambda^{n+2}, the kept Laplacian and the adjoint faces
+    float4 lc[MR], l2c[MR], lpn[MR], i0[MR], i1[MR];
+    float inx[MR][5];
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
-      tile_store<TY, R>(lq, vl);
+    for (int k = 0; k < MR; ++k) {
+      const int y = y0 + r0 + k;
+      const bool ok = act && y < D1;
+      lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);
+      l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+      lpn[k] = (ok && kept_lap && imaging) ? ld4(io.lap_n + base + (y * P2 + x)) : f4(0.f);
+      const bool faces = !Q && ok && !(col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1]);
+      i0[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x)) : f4(0.f);
+      i1[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x)) : f4(0.f);
+      const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) inx[k][j] = (faces && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
     }
-    if (p_tile) {
-      float4 vp[S::per_thread];
-      tile_load<TY, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
-      tile_store<TY, R>(lp, vp);
-    }
-    __syncthreads();
+    lds_barrier();  // the previous shot's reads of lq / lp are done
+#pragma unroll
+    for (int k = 0; k < S::per_thread; ++k) vl[k] = mul4(vm[k], vl[k]);
+    tile_store_w<S>(lq, vl);
+    if (p_tile) tile_store_w<S>(lp, vp);
+    lds_barrier();
     float4 qw[2 * R + 1], pw[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) {
@@ -1675,7 +1717,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
       pw[k] = p_tile ? lrow(lp, r0 + k, lane) : f4(0.f);
     }
 #pragma unroll
-    for (int k = 0; k < S::my_rows; ++k) {
+    for (int k = 0; k < MR; ++k) {
       const int y = y0 + r0 + k;
       qw[2 * R] = lrow(lq, r0 + k + 2 * R, lane);
       pw[2 * R] = p_tile ? lrow(lp, r0 + k + 2 * R, lane) : f4(0.f);
@@ -1689,42 +1731,39 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
         } else {
           dm = damp4_row<NW_PML>(ldc(g.damp[1], y), hx, dcol);
         }
-        const float4 lc = ld4(L1 + ro), l2c = ld4(io.lam2 + base + ro);
+        const float4 lcv = lc[k], l2v = l2c[k];
         const float zcy = Q ? 0.f : ldc(g.zc[1], y);
         const float zzr = g.dt2 * zcy;
         float4 rb, lam;
 #define NW_PML_ADJ(c)                                                                        \
   {                                                                                          \
-    rb.c = dm.a.c * lc.c;                                                                    \
-    const float pbar = -dm.b.c * (dm.a.c * l2c.c);                                           \
+    rb.c = dm.a.c * lcv.c;                                                                   \
+    const float pbar = -dm.b.c * (dm.a.c * l2v.c);                                           \
     lam.c = __builtin_fmaf(2.f - zzr * zcx.c, rb.c, pbar) + lapq.c;                          \
   }
         NW_PML_ADJ(x) NW_PML_ADJ(y) NW_PML_ADJ(z) NW_PML_ADJ(w)
 #undef NW_PML_ADJ
         float4 lapp = f4(0.f);
         if (kept_lap && imaging) {
-          lapp = ld4(io.lap_n + base + ro);
+          lapp = lpn[k];
         } else if (p_tile) {
           const float4 pn = lnb(lp, r0 + k + R, lane);
           lapp = stencil4<R>(g, pw, pn.x, pn.y, pn.z, pn.w);
         }
         if constexpr (!Q) {
-          const bool lq = col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1];
+          const bool lqr = col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1];
           // y faces of rows y and y + 1
           {
             const float4 qc = qw[R], qm = qw[R - 1], qp = qw[R + 1];
             const float ay0 = ldc(g.alpha[1], y), ay1 = ldc(g.alpha[1], y + 1);
             const float ky0 = ldc(g.kappa[1], y), ky1 = ldc(g.kappa[1], y + 1);
             const float zy0 = ldc(g.zf[1], y), zy1 = ldc(g.zf[1], y + 1);
-            const float4 i0 =
-                lq ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x));
-            const float4 i1 =
-                lq ? f4(0.f) : ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x));
+            const float4 a0 = i0[k], a1 = i1[k];
             float4 o0, o1;
 #define NW_PML_ADJ_Y(c)                                                                      \
   {                                                                                          \
     const float dq0 = qc.c - qm.c, dq1 = qp.c - qc.c;                                        \
-    const float t0 = i0.c - ih1 * dq0, t1 = i1.c - ih1 * dq1;                                \
+    const float t0 = a0.c - ih1 * dq0, t1 = a1.c - ih1 * dq1;                                \
     const float b0 = (zcx.c - zy0) * ky0, b1 = (zcx.c - zy1) * ky1;                          \
     lam.c = lam.c - (b1 * t1 - b0 * t0);                                                     \
     o0.c = ay0 * t0 - ih1 * dq0;                                                             \
@@ -1732,11 +1771,11 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   }
             NW_PML_ADJ_Y(x) NW_PML_ADJ_Y(y) NW_PML_ADJ_Y(z) NW_PML_ADJ_Y(w)
 #undef NW_PML_ADJ_Y
-            if (!lq) {
+            if (!lqr) {
               st4(io.psib_out[1] + fy + ((int64_t)y * D2 + x), o0);
               if (y == D1 - 1) st4(io.psib_out[1] + fy + ((int64_t)(y + 1) * D2 + x), o1);
             }
-            if (p_tile && !lq) {
+            if (p_tile && !lqr) {
               const float4 h0 = ld4(io.psi_half[1] + fy + ((int64_t)y * D2 + x));
               const float4 h1 = ld4(io.psi_half[1] + fy + ((int64_t)(y + 1) * D2 + x));
               const float4 w0 = ld4(io.psi_low[1] + fy + ((int64_t)y * D2 + x));
@@ -1751,13 +1790,11 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
           {
             const float4 qc = qw[R];
             const float qf[6] = {qn.y, qc.x, qc.y, qc.z, qc.w, qn.z};  // q at x-1 .. x+4
-            const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
             float t[5], dq[5], b[5];
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
-              const float in = (!lq && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
               dq[j] = qf[j + 1] - qf[j];
-              t[j] = in - ih2 * dq[j];
+              t[j] = inx[k][j] - ih2 * dq[j];
               b[j] = (zcy - px.zf[j]) * px.k[j];
             }
             lam.x = lam.x - (b[1] * t[1] - b[0] * t[0]);
@@ -1765,12 +1802,12 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
             lam.z = lam.z - (b[3] * t[3] - b[2] * t[2]);
             lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
             float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
-            if (!lq) {
+            if (!lqr) {
 #pragma unroll
               for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
               if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
             }
-            if (p_tile && !lq) {
+            if (p_tile && !lqr) {
               const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
               const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
               float hw[5];
@@ -1801,7 +1838,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   if (imaging) {
     double* G = io.grad + (int64_t)tl.grp * g.cells;
 #pragma unroll
-    for (int k = 0; k < S::my_rows; ++k) {
+    for (int k = 0; k < MR; ++k) {
       const int y = y0 + r0 + k;
       if (act && y < D1) {
         double2* gp = reinterpret_cast<double2*>(G + (y * P2 + x));
@@ -1818,9 +1855,11 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
 }
 
 template <int R, int TY>
-__global__ void __launch_bounds__(256) k3_adjoint_pml(Geo<float> g, AdjIO<float> io,
-                                                      int shots, int spw, int xcd_map) {
-  using S = TileShape<TY, R>;
+__global__ void __launch_bounds__(64 * AdjWaves<TY>::waves) k3_adjoint_pml(Geo<float> g,
+                                                                          AdjIO<float> io,
+                                                                          int shots, int spw,
+                                                                          int xcd_map) {
+  using S = AdjShape<TY, R>;
   __shared__ __align__(16) float lq[S::rows * kLW];  // q = (m a_eff) lambda^{n+1}
   __shared__ __align__(16) float lp[S::rows * kLW];  // p^n
   const Tile tl =
@@ -2226,7 +2265,8 @@ static void launch_adj_t(const Geo<float>& g, const AdjIO<float>& io, int shots,
                          hipStream_t st) {
   if constexpr (ABS == NW_PML) {
     hipLaunchKernelGGL((k3_adjoint_pml<R, pml_rows<TY>()>),
-                       dim3(grid_tiles3(g, pml_rows<TY>(), shots, spw)), dim3(256), 0, st, g, io,
+                       dim3(grid_tiles3(g, pml_rows<TY>(), shots, spw)),
+                       dim3(64 * AdjWaves<pml_rows<TY>()>::waves), 0, st, g, io,
                        shots, spw, xm & 1);
   } else if ((xm & 4) && TY >= 8 && TY <= 32) {
     const int tiles = block_tiles(adj_cols((int)g.D[2]), div_up((int)g.D[1], lds_rows<TY>()),
