@@ -387,30 +387,69 @@ def misfit_bench(rows, n_t, dt_obs, f_max, dev, reps=5):
     }
 
 
-def pat3d_bench(args, dev) -> dict:
-    """The 3-D photoacoustic operator (192^3 + 10-cell sponge = 212^3, order 2, 4096 detectors on
-    the top face, 1500 steps, speed 1500 m/s +-5 % smooth) forward + adjoint in float32:
-    throughput, and the roofline of its dominant kernel from HIP events."""
-    import torch
+def pat3d_config(n=192, dx=1e-4, seed=2, n_det=4096, n_abs=20):
+    """Inputs of the 3-D photoacoustic configuration (host, float64): 192^3 cells of 0.1 mm,
+    sound speed 1500 m/s with +-5 % smooth seeded heterogeneity (Gaussian-filtered N(0,1),
+    sigma 6 cells), initial pressure of ``n_abs`` seeded ellipsoidal absorbers (amplitudes
+    U[0.5, 1], semi-axes U[0.4, 1.2] mm, random orientations, tanh edges 0.15 mm wide, summed
+    and clipped to 1), and ``n_det`` detectors on a hemispherical cap: radius 0.45 L around the
+    centre of the top face (depth = last axis), reaching into the volume, sampled by a
+    Fibonacci lattice (equal-area bands in depth, golden-angle azimuths). The absorbers lie
+    inside the cap. Returns ``(L, c, p0, detectors)``."""
     from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    L = n * dx
+    pert = gaussian_filter(rng.standard_normal((n, n, n)), 6.0, mode="wrap")
+    c = 1500.0 * (1.0 + 0.05 * pert / np.abs(pert).max())
+    R = 0.45 * L
+    centre = np.array([0.5 * L, 0.5 * L, 0.0])
+    k = np.arange(n_det)
+    cos_t = (k + 0.5) / n_det
+    sin_t = np.sqrt(1.0 - cos_t**2)
+    phi = np.pi * (3.0 - math.sqrt(5.0)) * k
+    det = centre + R * np.stack([sin_t * np.cos(phi), sin_t * np.sin(phi), cos_t], 1)
+    ax = (np.arange(n) + 0.5) * dx
+    p0 = np.zeros((n, n, n))
+    width = 1.5 * dx
+    for _ in range(n_abs):
+        while True:  # a centre in the cap, away from its rim
+            q = centre + rng.uniform(-0.7 * R, 0.7 * R, 3) * np.array([1.0, 1.0, 0.0])
+            q[2] = rng.uniform(0.15 * R, 0.75 * R)
+            if np.linalg.norm(q - centre) < 0.7 * R:
+                break
+        semi = rng.uniform(4.0, 12.0, 3) * dx
+        rot, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        amp = rng.uniform(0.5, 1.0)
+        reach = semi.max() + 4 * width
+        sl = [slice(max(0, int((q[a] - reach) / dx)), min(n, int((q[a] + reach) / dx) + 2))
+              for a in range(3)]
+        X, Y, Z = np.meshgrid(ax[sl[0]] - q[0], ax[sl[1]] - q[1], ax[sl[2]] - q[2],
+                              indexing="ij")
+        body = np.einsum("ij,...j->...i", rot.T, np.stack([X, Y, Z], -1)) / semi
+        dist = (np.sqrt((body**2).sum(-1)) - 1.0) * semi.min()  # about the signed distance
+        p0[tuple(sl)] += amp * 0.5 * (1.0 - np.tanh(dist / width))
+    return L, c, np.minimum(p0, 1.0), det.tolist()
+
+
+def pat3d_bench(args, dev) -> dict:
+    """The 3-D photoacoustic operator (192^3 + 10-cell sponge = 212^3, order 2, 1500 steps; the
+    inputs of ``pat3d_config``: 20 ellipsoidal absorbers, 4096 detectors on a hemispherical cap)
+    forward + adjoint in float32: throughput, and the roofline of its dominant kernel from HIP
+    events."""
+    import torch
 
     from nefi_amd import _native as nat
     from nefi_amd.domain import Domain
     from nefi_amd.physics.wave import WaveInitialConditionOperator
 
     n, dx = 192, 1e-4
-    L = n * dx
-    rng = np.random.default_rng(2)
-    pert = gaussian_filter(rng.standard_normal((n, n, n)), 6.0, mode="wrap")
-    c = 1500.0 * (1.0 + 0.05 * pert / np.abs(pert).max())
-    det = [[(i + 0.5) * L / 64, (j + 0.5) * L / 64, 0.5 * dx] for i in range(64)
-           for j in range(64)]
+    L, c, p0_np, det = pat3d_config(n, dx)
     c_max = float(c.max())
     op = WaveInitialConditionOperator(Domain((n, n, n), ((0.0, L),) * 3), det, n_t=1501,
                                       dt_obs=0.3 * dx / c_max, c=torch.tensor(c), c_max=c_max,
                                       order=2, absorbing="sponge", absorb_width=10 * dx)
-    p0 = torch.tensor(gaussian_filter(rng.standard_normal((n, n, n)), 3.0, mode="wrap"),
-                      dtype=torch.float32, device=dev, requires_grad=True)
+    p0 = torch.tensor(p0_np, dtype=torch.float32, device=dev, requires_grad=True)
     w = torch.randn(len(det), 1501, device=dev, generator=torch.Generator(dev).manual_seed(3))
 
     def step():
@@ -448,7 +487,8 @@ def pat3d_bench(args, dev) -> dict:
         "ms_per_step": round(elapsed * 1e3, 2),
         "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"3-D PAT, 192^3 + 10-cell sponge = {padded[0]}^3, order 2, "
-                               f"4096 detectors, {op.n_steps} steps, forward + adjoint"},
+                               f"20 ellipsoidal absorbers, 4096 detectors on a hemispherical cap, "
+                               f"{op.n_steps} steps, forward + adjoint"},
         "roofline": {"bound": "hbm",
                      "kernel": (f"k3d_{dom}<order 2, sponge> (2.5-D tiles)" if fast
                                 else f"k_{dom}<float, order 2, sponge> (generic)"),

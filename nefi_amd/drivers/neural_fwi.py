@@ -136,6 +136,9 @@ def run(iters: int = 500, n: int = 512, shots: int = 32, receivers: int = 256,
     ax = (torch.arange(n, dtype=torch.float32, device=device) + 0.5) / n * 2.0 - 1.0
     coords = torch.stack(torch.meshgrid(ax, ax, indexing="ij"), -1).reshape(-1, 2)
     field = NeuralSpeed().to(device)
+    with torch.no_grad():
+        c_start = field(coords, 0.0).reshape(n, n)
+        start_err = float(torch.linalg.norm(c_start - c_true) / torch.linalg.norm(c_true))
     opt = torch.optim.AdamW(field.parameters(), lr=lr, weight_decay=1e-4)
     history, times = [], []
     for step in range(iters):
@@ -161,7 +164,8 @@ def run(iters: int = 500, n: int = 512, shots: int = 32, receivers: int = 256,
     steady = times[1:] if len(times) > 1 else times
     s_iter = float(np.mean(steady))
     cells = op.grid.padded_shape[0] * op.grid.padded_shape[1]
-    return {"misfit": history, "model_rel_error": model_err, "s_per_iter": s_iter,
+    return {"misfit": history, "model_rel_error": model_err,
+            "model_rel_error_start": start_err, "s_per_iter": s_iter,
             "gcell_updates_per_s": cells * op.n_steps * shots / s_iter / 1e9,
             "padded": list(op.grid.padded_shape), "steps": op.n_steps, "shots": shots}
 
@@ -176,6 +180,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     res = run(args.iters, args.n, args.shots, args.receivers, args.nt)
     res["misfit_first_last"] = [res["misfit"][0], res["misfit"][-1]]
+    res["misfit_every_50"] = res["misfit"][::50]
     del res["misfit"]
     print(json.dumps(res), flush=True)
     return 0

@@ -4,8 +4,9 @@
   gradient of J = ½‖d(c) − d_obs‖² at the linear background against central finite
   differences along 16 seeded unit directions, and the float32 gradient (fast kernels) against
   the float64 one.
-* Config 4 (3-D photoacoustic, 192³ + 10-cell sponge = 212³, 4096 detectors, 1500 steps): the
-  adjoint dot-product test ⟨A p0, w⟩ = ⟨p0, Aᵀ w⟩ of the initial-value operator, float32
+* Config 4 (3-D photoacoustic, 192³ + 10-cell sponge = 212³, initial pressure of 20 seeded
+  ellipsoidal absorbers, 4096 detectors on a hemispherical cap, 1500 steps; ``bench.pat3d_config``):
+  the adjoint dot-product test ⟨A p0, w⟩ = ⟨p0, Aᵀ w⟩ of the initial-value operator, float32
   traces against float64, and the float64 p0 gradient of a quadratic misfit against central
   differences along 4 seeded directions.
 * Config 5 (512² + 20-cell sponge = 552², 32 shots, 3000 steps, layered model): the float32
@@ -92,20 +93,16 @@ def test_config2_gradient_against_central_differences():
 
 
 def test_config4_pat3d_adjoint_dot_product_and_float32_traces():
+    import bench
+
     n, dx = 192, 1e-4
-    L = n * dx
-    rng = np.random.default_rng(2)
-    cpert = gaussian_filter(rng.standard_normal((n, n, n)), 6.0, mode="wrap")
-    c = 1500.0 * (1.0 + 0.05 * cpert / np.abs(cpert).max())
-    det = [[(i + 0.5) * L / 64, (j + 0.5) * L / 64, 0.5 * dx] for i in range(64)
-           for j in range(64)]
+    L, c, p0, det = bench.pat3d_config(n, dx)  # 20 ellipsoids, 4096 detectors on a cap
     c_max = float(c.max())
     op = WaveInitialConditionOperator(Domain((n, n, n), ((0.0, L),) * 3), det, n_t=1501,
                                       dt_obs=0.3 * dx / c_max, c=torch.tensor(c), c_max=c_max,
                                       order=2, absorbing="sponge", absorb_width=10 * dx)
     print(f"[case config 4: padded {op.grid.padded_shape}, {op.n_steps} steps, "
           f"{len(det)} detectors]", flush=True)
-    p0 = gaussian_filter(rng.standard_normal((n, n, n)), 3.0, mode="wrap")
     x = torch.tensor(p0, device=DEV, dtype=torch.float32, requires_grad=True)
     w = torch.randn(len(det), 1501, device=DEV, generator=torch.Generator(DEV).manual_seed(3))
     t0 = time.perf_counter()
