@@ -402,3 +402,97 @@ def test_sensor_bandwidth_attributes_and_resolution_copies():
     assert ideal.fidelity_tag == "wave-leapfrog-o2-sponge" and ideal.f_cutoff is None
     x = torch.ones(2, 5)
     assert ideal._filter(x) is x
+
+
+def _blocked_lds_phases(waves, region_rows, R, levels, shots, quiet):
+    """The LDS accesses of one k4_steps workgroup (nefi_wave_2d.hip, k4_tile) as barrier
+    intervals: per interval, per wave, the (buffer, row) sets read and written, plus the rows
+    an asynchronous region fill (quiet tiles) may write until the next shot's full wait.
+    Row numbers are LDS rows (R zero rows, then the region's computed rows)."""
+    mr = -(-region_rows // waves)
+    crow = waves * mr
+    lds_w4 = 66  # float4 per LDS row (kLW / 4)
+    threads = 64 * waves
+    n4 = crow * lds_w4
+
+    def row(b, r, cols=range(lds_w4)):  # the float4 elements of LDS row r of buffer b
+        return {(b, r, c) for c in cols}
+
+    def tile_elems(b, wave):  # elements a wave's lanes store when the region is stored by threads
+        out = set()
+        for lane in range(64):
+            i = wave * 64 + lane
+            while i < n4:
+                out.add((b, R + i // lds_w4, i % lds_w4))
+                i += threads
+        return out
+
+    phases = []  # list of (reads[w], writes[w], async_rows or None)
+    buf = lambda s: (s % 2) if quiet else 0  # noqa: E731
+    for s in range(shots):
+        b = buf(s)
+        if not quiet:  # synchronous store of the region after a barrier
+            phases.append(([set() for _ in range(waves)],
+                           [tile_elems(b, w) for w in range(waves)], None))
+        # after the barrier: cur rows, then level 1 reads (windows), and for quiet tiles the
+        # next shot's fill issued into the other buffer
+        nxt = None
+        if quiet and s + 1 < shots:
+            nxt = set().union(*(tile_elems(buf(s + 1), w) for w in range(waves)))
+        for lvl in range(1, levels + 1):
+            reads = []
+            for w in range(waves):
+                c0 = w * mr
+                rr = set()
+                for k in range(mr if lvl == 1 else 0):
+                    rr |= row(b, R + c0 + k)
+                for j in range(mr + 2 * R):  # the window rows, x-neighbours included
+                    rr |= row(b, c0 + j)
+                reads.append(rr)
+            phases.append((reads, [set() for _ in range(waves)], nxt))
+            if lvl < levels:  # level barrier, write the new level, barrier, tap reads
+                writes = []
+                for w in range(waves):
+                    ww = set()
+                    for k in range(mr):  # lane l writes float4 l + 1 of its rows
+                        ww |= row(b, R + w * mr + k, range(1, 65))
+                    writes.append(ww)
+                phases.append(([set() for _ in range(waves)], writes, nxt))
+                taps = set()
+                for r in range(R * levels, crow - R * levels):
+                    taps |= row(b, R + r)
+                phases.append(([set(taps) for _ in range(waves)], [set() for _ in range(waves)],
+                               nxt))
+        # the next shot starts with a full wait for its fill and a barrier
+    return phases
+
+
+@pytest.mark.parametrize("waves", [4, 8, 12, 16])
+@pytest.mark.parametrize("shots", [1, 2, 3, 4])
+@pytest.mark.parametrize("quiet", [False, True])
+def test_blocked_forward_lds_protocol_has_no_races(waves, shots, quiet):
+    """Between two barriers of the temporally blocked forward kernel no wavefront writes an LDS
+    row another wavefront reads or writes, and a quiet tile's asynchronous fill of the next
+    shot's region never lands in the buffer being read: for any number of wavefronts and shots
+    per workgroup (32-row tiles, depth 4, order 4: 48 region rows), including the 4-wavefront,
+    2-shot workgroup that once gave wrong results (the protocol is the same source for every
+    wave count, so that failure did not come from the barrier and level ordering)."""
+    R, T = 2, 4
+    for reads, writes, fill in _blocked_lds_phases(waves, 32 + 2 * R * T, R, T, shots, quiet):
+        for w in range(waves):
+            for v in range(waves):
+                if v != w:
+                    assert not (writes[w] & (reads[v] | writes[v])), (w, v)
+            if fill:
+                assert not (fill & (reads[w] | writes[w]))
+
+
+def test_blocked_forward_lds_protocol_check_finds_a_missing_barrier():
+    """The same check on a model with the barrier between a level's reads and the next level's
+    writes removed (the two intervals merged) reports the race."""
+    phases = _blocked_lds_phases(8, 48, 2, 4, 2, False)
+    i = next(k for k, (r, w, _) in enumerate(phases) if any(r) and not any(w)
+             and k + 1 < len(phases) and any(phases[k + 1][1]))
+    reads = phases[i][0]
+    writes = phases[i + 1][1]
+    assert any(writes[w] & reads[v] for w in range(8) for v in range(8) if v != w)
