@@ -217,7 +217,18 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   not cover report 0 and refuse 1.
  * NW_OPT_FAST_F64: 1 (default where it covers the plan: 2-D or 3-D, float64, sponge/Cerjan/
  *   none, padded width and row pitch even) runs the LDS-tiled float64 forward step (2.5-D
- *   tiles in 3-D); 0 the generic one. Plans it does not cover report 0 and refuse 1. */
+ *   tiles in 3-D); 0 the generic one. Plans it does not cover report 0 and refuse 1.
+ * NW_OPT_WORK_QUEUE: how temporally blocked forward launches (NW_OPT_TIME_BLOCK > 1) are
+ *   scheduled. A work queue runs as many workgroups as the device holds at once, each taking
+ *   tiles (and, at the end, single shots of tiles) from per-XCD queues until all are empty;
+ *   otherwise there is one workgroup per tile. 1 (default): the queue for calls that fill no
+ *   keep buffer (a keep sweep may run beside reverse steps on another stream, which the
+ *   queue's resident workgroups would keep off the CUs); 2: for every blocked call; 0: never.
+ *   Results are identical. The queue's items are built when the forward tiles, the depth or
+ *   this option are set (which waits for the device), and its counters are the plan's: calls
+ *   on one plan must not run concurrently (as for its blocking wavefields).
+ * NW_OPT_WORK_QUEUE_ITEMS (read only): items of the plan's current work queue; 0 when blocked
+ *   launches take one workgroup per tile (queue off, no blocking, or no device). */
 enum nw_option {
   NW_OPT_XCD_MAP = 0,
   NW_OPT_ROWS_PER_ITER = 1,
@@ -229,7 +240,9 @@ enum nw_option {
   NW_OPT_KEEP_LAP = 7,
   NW_OPT_TIME_BLOCK_ADJOINT = 8,
   NW_OPT_FAST_3D = 9,
-  NW_OPT_FAST_F64 = 10
+  NW_OPT_FAST_F64 = 10,
+  NW_OPT_WORK_QUEUE = 11,
+  NW_OPT_WORK_QUEUE_ITEMS = 12
 };
 int32_t nw_plan_set_option(nw_plan* plan, int32_t option, int32_t value);
 int32_t nw_plan_get_option(const nw_plan* plan, int32_t option, int32_t* value);

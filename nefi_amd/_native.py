@@ -218,6 +218,8 @@ OPT_KEEP_LAP = 7
 OPT_TIME_BLOCK_ADJOINT = 8
 OPT_FAST_3D = 9
 OPT_FAST_F64 = 10
+OPT_WORK_QUEUE = 11
+OPT_WORK_QUEUE_ITEMS = 12
 
 
 def profile_enable(on: bool) -> None:

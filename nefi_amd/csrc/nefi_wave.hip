@@ -565,6 +565,14 @@ struct nw_plan {
   // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
   int tblock_adj;
   float* injv;
+  // work queue of the blocked forward launches (NW_OPT_WORK_QUEUE): items for the current
+  // forward tiles and depth (built when they are set), counters, and the device's CU count
+  int work_queue;
+  int n_cu;
+  int4* wq_items;
+  int64_t wq_cap;  // items the buffer holds
+  int wq_key[3];   // rows, depth, shots per workgroup of the items (0: none built)
+  nw::BlockQueue wq;
   // hipGraph capture of whole nw_forward / nw_adjoint calls (NW_OPT_GRAPH)
   int graph_mode;
   hipStream_t capture_stream;
@@ -886,8 +894,14 @@ static int forward_blocked(nw_plan* P, const float* m, void** state, int n_begin
       io.rec_idx = d.rec_idx;
     }
     const int pid = prof_begin(NW_K_STEPS, st, L);
+    // keep sweeps run beside reverse steps in an overlapped reverse sweep, where persistent
+    // workgroups would hold their CUs until the queue is empty: by default they take one
+    // workgroup per tile
+    const bool queued = (P->work_queue == 2 || (P->work_queue == 1 && !keeping)) &&
+                        P->wq_key[0] == P->rows[0] && P->wq_key[1] == P->tblock &&
+                        P->wq_key[2] == P->spw[0];
     fast2d_steps(P->gf, d.order, d.absorbing, io, d.n_shots, P->rows[0], P->tblock, P->spw[0],
-                 P->xcd_map, st);
+                 P->xcd_map, queued ? &P->wq : nullptr, st);
     prof_end(pid, st);
     NW_TRY(check_launch("k4_steps"));
     if (io.tapbuf) {
@@ -1408,6 +1422,18 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->fast64 = fast2d64_supported(p->gd, d.absorbing, d.dtype) ? 1 : 0;
   p->tblock_adj = 1;
   p->injv = nullptr;
+  p->work_queue = 1;
+  p->n_cu = 0;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      p->n_cu = 0;  // no device: blocked launches use one workgroup per tile
+  }
+  p->wq_items = nullptr;
+  p->wq_cap = 0;
+  p->wq_key[0] = p->wq_key[1] = p->wq_key[2] = 0;
+  p->wq = nw::BlockQueue{};
   p->graph_mode = 0;
   p->capture_stream = nullptr;
   p->graph_serial = 0;
@@ -1428,6 +1454,8 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
   for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
     if (b) (void)hipFree(b);
+  if (plan->wq_items) (void)hipFree(plan->wq_items);
+  if (plan->wq.ctr) (void)hipFree(plan->wq.ctr);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)
     if (nw::g_plans[i] == plan) {
       nw::g_plans.erase(nw::g_plans.begin() + i);
@@ -1442,6 +1470,47 @@ int64_t nw_grad_slices(const nw_plan* P) {
   return P->d.n_shots;
 }
 
+// (Re)builds the work queue of the blocked forward launches for the plan's current forward
+// tiles and depth, or marks it unused (the launches then take one workgroup per tile). Runs
+// when tiles or options are set, never inside a captured call: it waits for the device (a
+// launch in flight may still read the previous items) and copies the items synchronously.
+static int build_queue(nw_plan* P) {
+  P->wq_key[0] = P->wq_key[1] = P->wq_key[2] = 0;
+  const nw_desc& d = P->d;
+  if (!P->work_queue || P->n_cu <= 0 || d.dtype != NW_F32 || d.absorbing == NW_PML ||
+      P->tblock < 2 || P->rows[0] <= 0 || !fast2d_block_supported(P->rows[0], P->tblock))
+    return NW_OK;
+  std::vector<int4> items;
+  int off[kQueues + 1], per_cu = 1;
+  // the last tiles of every queue go as single shots: about half a queue's CUs' worth
+  const int split_tail = std::max(0, P->n_cu / kQueues / 2);
+  NW_TRY(fast2d_steps_queue_items(P->gf, d.order, d.absorbing, P->rows[0], P->tblock,
+                                  d.n_shots, P->spw[0], split_tail, items, off, &per_cu));
+  NW_HIP(hipDeviceSynchronize());
+  if ((int64_t)items.size() > P->wq_cap) {
+    if (P->wq_items) NW_HIP(hipFree(P->wq_items));
+    P->wq_items = nullptr;
+    P->wq_cap = 0;
+    NW_HIP(hipMalloc((void**)&P->wq_items, items.size() * sizeof(int4)));
+    P->wq_cap = (int64_t)items.size();
+  }
+  NW_HIP(hipMemcpy(P->wq_items, items.data(), items.size() * sizeof(int4),
+                   hipMemcpyHostToDevice));
+  if (!P->wq.ctr) {
+    NW_HIP(hipMalloc((void**)&P->wq.ctr, (kQueues + 1) * sizeof(int)));
+    NW_HIP(hipMemset(P->wq.ctr, 0, (kQueues + 1) * sizeof(int)));
+    NW_HIP(hipDeviceSynchronize());
+  }
+  P->wq.items = P->wq_items;
+  for (int j = 0; j <= kQueues; ++j) P->wq.off[j] = off[j];
+  P->wq.grid = (int)std::min<int64_t>((int64_t)P->n_cu * per_cu, (int64_t)items.size());
+  if (P->wq.grid < 1) return NW_OK;
+  P->wq_key[0] = P->rows[0];
+  P->wq_key[1] = P->tblock;
+  P->wq_key[2] = P->spw[0];
+  return NW_OK;
+}
+
 int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_per_group) {
   if (!P || kind < 0 || kind > 1) return fail(NW_ERR_ARG, "nw_plan_set_tiles: bad plan or kind");
   drop_graphs(P);
@@ -1449,7 +1518,7 @@ int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_
     P->rows[kind] = 0;
     P->spw[kind] = 1;
     P->lds_tiles[kind] = 0;
-    return NW_OK;
+    return kind == 0 ? build_queue(P) : NW_OK;
   }
   if (!fast2d_supported(P->gf, P->d.absorbing, P->d.dtype))
     return fail(NW_ERR_UNSUPPORTED, "nw_plan_set_tiles: the fast 2-D kernels do not cover this plan");
@@ -1460,7 +1529,7 @@ int32_t nw_plan_set_tiles(nw_plan* P, int32_t kind, int32_t rows, int32_t shots_
   P->rows[kind] = rows;
   P->spw[kind] = shots_per_group;
   if (rows < 8 || rows > 32) P->lds_tiles[kind] = 0;
-  return NW_OK;
+  return kind == 0 ? build_queue(P) : NW_OK;
 }
 
 int32_t nw_plan_get_tiles(const nw_plan* P, int32_t kind, int32_t* rows, int32_t* shots_per_group) {
@@ -1527,7 +1596,12 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
       NW_HIP(hipMalloc((void**)&P->injv, (size_t)inj * sizeof(float)));
     }
     (option == NW_OPT_TIME_BLOCK ? P->tblock : P->tblock_adj) = value;
-    return NW_OK;
+    return option == NW_OPT_TIME_BLOCK ? build_queue(P) : NW_OK;
+  }
+  if (option == NW_OPT_WORK_QUEUE) {
+    if (value < 0 || value > 2) return fail(NW_ERR_ARG, "work queue mode must be 0, 1 or 2");
+    P->work_queue = value;
+    return build_queue(P);
   }
   return fail(NW_ERR_ARG, "nw_plan_set_option: unknown option");
 }
@@ -1572,6 +1646,14 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
   }
   if (option == NW_OPT_FAST_F64) {
     *value = P->fast64;
+    return NW_OK;
+  }
+  if (option == NW_OPT_WORK_QUEUE) {
+    *value = P->work_queue;
+    return NW_OK;
+  }
+  if (option == NW_OPT_WORK_QUEUE_ITEMS) {
+    *value = P->wq_key[0] ? P->wq.off[kQueues] : 0;
     return NW_OK;
   }
   return fail(NW_ERR_ARG, "nw_plan_get_option: unknown option");

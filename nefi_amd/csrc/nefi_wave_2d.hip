@@ -16,8 +16,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
+#include <vector>
 
 #include "nefi_wave.h"
 #include "nefi_wave_internal.h"
@@ -865,13 +867,23 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Issue the loads of one region of a quiet tile (every element inside the grid) straight into
 // LDS rows R.. of `buf` (global_load_lds: lane l of a wavefront lands 16 bytes after lane l-1,
 // from the wavefront's base address). Completion is awaited with s_waitcnt before use.
+// threadIdx.x as a value the compiler cannot see through: per-thread index arithmetic of the
+// tile loads (row and column of each of a thread's float4 elements) is then recomputed where
+// it is used instead of being hoisted out of the shot and item loops, where its long live
+// ranges pushed the blocked kernels over their register budget (scratch spills).
+__device__ __forceinline__ int thread_index() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <typename S, int R>
 __device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, int P2,
                                              float* buf) {
-  const int wbase = threadIdx.x & ~63;
+  const int wbase = thread_index() & ~63;
 #pragma unroll
   for (int k = 0; k < S::per_thread; ++k) {
-    const int i = threadIdx.x + S::threads * k;
+    const int i = thread_index() + S::threads * k;
     if (S::threads * (k + 1) <= S::n4 || i < S::n4) {
       const int row = i / (kLW / 4), c4 = i % (kLW / 4);
       const float* src = P + ((ry0 + row) * P2 + rx0 - 4 + 4 * c4);
@@ -893,14 +905,16 @@ __device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, i
 // margin). The x-neighbour reads of lanes 31 and 32 then see the other tile's edge column;
 // like the margins, that only reaches region columns within R T of the region's edge, which
 // the output columns are not (hx >= R T).
+// The tile advances shots [s_first, s_end) (tl.grp is not read); `shots` is the plan's count.
 template <int R, int ABS0, int TY, int T, bool Q, bool SP = false>
 __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, int shots,
-                                        int spw, const Tile& tl, float* lpa, float* lpb) {
+                                        int s_first, int s_end, const Tile& tl, float* lpa,
+                                        float* lpb) {
   static_assert(!(Q && SP), "split tiles lie on the grid's edge");
   constexpr int ABS = Q ? NW_NONE : ABS0;
   using S = BlockShape<TY, R, T>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int lane = threadIdx.x & 63, wave = wave_index();
+  const int lane = thread_index() & 63, wave = wave_index();
   const int hl = SP ? (lane & 31) : lane;    // lane within its tile
   const int ryA = tl.ry * TY - S::hy;        // grid row of region row 0 (upper tile if SP)
   const int ry0 = SP && lane >= 32 ? ryA + TY : ryA;  // ... of this lane's tile
@@ -933,8 +947,6 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
                                                 : f4(0.f);
     }
   };
-  const int s_first = tl.grp * spw;
-  const int s_end = min(shots, s_first + spw);
   float4 pvn[Q ? S::mr : 1];
   if constexpr (Q) {
     region_async<S, R>(io.p + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
@@ -957,7 +969,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
       const float* P = io.p + base;
 #pragma unroll
       for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + S::threads * k;
+        const int i = thread_index() + S::threads * k;
         const int row = i / (kLW / 4), c4 = i % (kLW / 4);
         const bool lower = SP && c4 >= kLW / 8;  // the lower tile's half of the LDS row
         const int gy = (lower ? ryA + TY : ryA) + row;
@@ -970,7 +982,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
       __syncthreads();  // the previous shot's reads of lp are done
 #pragma unroll
       for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + S::threads * k;
+        const int i = thread_index() + S::threads * k;
         if (S::threads * (k + 1) <= S::n4 || i < S::n4)
           *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
       }
@@ -988,7 +1000,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     const int tap0 = io.tapbuf ? io.rec_first[s] * io.taps : 0;
 #pragma unroll
     for (int c = 0; c < BlockWaves<TY>::tap_chunks; ++c) {
-      const int e = threadIdx.x + S::threads * c;
+      const int e = thread_index() + S::threads * c;
       toff[c] = -1;
       if (e < n_tap) {
         const int idx = io.rec_idx[tap0 + e];
@@ -1088,7 +1100,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
           float* tb = io.tapbuf + ((int64_t)(lvl - 1) * shots + s) * io.tap_stride;
 #pragma unroll
           for (int c = 0; c < BlockWaves<TY>::tap_chunks; ++c)
-            if (toff[c] >= 0) tb[threadIdx.x + S::threads * c] = lp[toff[c]];
+            if (toff[c] >= 0) tb[thread_index() + S::threads * c] = lp[toff[c]];
         }
       }
       float* out = io.out[lvl];
@@ -1114,32 +1126,115 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   }
 }
 
-template <int R, int ABS, int TY, int T>
-__global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g, BlockIO io, int shots,
-                                                          int spw, int xcd_map) {
-  using S = BlockShape<TY, R, T>;
-  __shared__ __align__(16) float lpa[S::lrows * kLW];
-  __shared__ __align__(16) float lpb[S::lrows * kLW];
-  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
-  const BlockCols cols = block_cols<S>(D2);
-  const Tile tl = tile_of_cols(cols, div_up(D1, TY), div_up(shots, spw), xcd_map);
-  if (!tl.valid) return;  // uniform over the workgroup
+// The R zero rows above and below the region in both LDS buffers (the stencil of the region's
+// first and last rows reads them); tiles never write them.
+template <class S, int R>
+__device__ __forceinline__ void zero_margin_rows(float* lpa, float* lpb) {
   for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
     lpa[i] = 0.f;
     lpa[(S::lrows - R) * kLW + i] = 0.f;
     lpb[i] = 0.f;
     lpb[(S::lrows - R) * kLW + i] = 0.f;
   }
+}
+
+// Shots [s_first, s_end) of tile tl, with the variant its position selects: split column,
+// quiet (region inside the grid and the box of neutral damping) or boundary.
+template <int R, int ABS, int TY, int T>
+__device__ __forceinline__ void k4_any_tile(const Geo<float>& g, const BlockIO& io, int shots,
+                                            int s_first, int s_end, const Tile& tl,
+                                            const BlockCols& cols, float* lpa, float* lpb) {
+  using S = BlockShape<TY, R, T>;
+  const int D1 = (int)g.D[1], D2 = (int)g.D[2];
   const int ry0 = tl.ry * TY - S::hy, rx0 = tl.cx * S::out_w - S::hx;
   const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
                      ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&
                      rx0 >= g.quiet[2][0] && rx0 + 256 <= g.quiet[2][1];
   if (cols.split && tl.cx == cols.full)
-    k4_tile<R, ABS, TY, T, false, true>(g, io, shots, spw, tl, lpa, lpb);
+    k4_tile<R, ABS, TY, T, false, true>(g, io, shots, s_first, s_end, tl, lpa, lpb);
   else if (quiet)
-    k4_tile<R, ABS, TY, T, true>(g, io, shots, spw, tl, lpa, lpb);
+    k4_tile<R, ABS, TY, T, true>(g, io, shots, s_first, s_end, tl, lpa, lpb);
   else
-    k4_tile<R, ABS, TY, T, false>(g, io, shots, spw, tl, lpa, lpb);
+    k4_tile<R, ABS, TY, T, false>(g, io, shots, s_first, s_end, tl, lpa, lpb);
+}
+
+template <int R, int ABS, int TY, int T>
+__global__ void __launch_bounds__(BlockWaves<TY>::threads) k4_steps(Geo<float> g, BlockIO io, int shots,
+                                                          int spw, int xcd_map) {
+  using S = BlockShape<TY, R, T>;
+  __shared__ __align__(16) float lpa[S::lrows * kLW];
+  __shared__ __align__(16) float lpb[S::lrows * kLW];
+  const BlockCols cols = block_cols<S>((int)g.D[2]);
+  const Tile tl = tile_of_cols(cols, div_up((int)g.D[1], TY), div_up(shots, spw), xcd_map);
+  if (!tl.valid) return;  // uniform over the workgroup
+  zero_margin_rows<S, R>(lpa, lpb);
+  const int s_first = tl.grp * spw;
+  k4_any_tile<R, ABS, TY, T>(g, io, shots, s_first, min(shots, s_first + spw), tl, cols, lpa,
+                             lpb);
+}
+
+// k4_steps as a work queue. One launch has as many workgroups as fit on the device at once;
+// each takes (tile, shot range) items until every queue is empty, so no workgroup waits for
+// another and the device is not idle between tiles while the dispatcher starts the next
+// workgroup (about 3 us per tile on MI355X), nor in a tail of a few long tiles. Queue j
+// holds the tiles k4_steps's XCD map gives XCD j, in the same order (row strips that share
+// halo rows run on one XCD and share its L2), the last ones split into single shots. A
+// workgroup serves the queue of the XCD it runs on (hardware register XCC_ID), then takes
+// from the others in turn. The items' cells, arithmetic and outputs are those of k4_steps.
+// Counters: ctr[j] is the next item of queue j, ctr[kXcds] counts the workgroups that found
+// every queue empty; the last of them sets all back to zero for the next launch, so launches
+// that share ctr must not overlap (one plan's launches are stream ordered).
+struct BlockQueues {
+  int off[kXcds + 1];  // items of queue j: [off[j], off[j + 1])
+};
+static_assert(kQueues == kXcds, "one work queue per XCD");
+
+template <int R, int ABS, int TY, int T>
+__global__ void __launch_bounds__(BlockWaves<TY>::threads)
+    k4_queue(Geo<float> g, BlockIO io, int shots, const int4* __restrict__ items,
+             BlockQueues q, int* ctr) {
+  using S = BlockShape<TY, R, T>;
+  __shared__ __align__(16) float lpa[S::lrows * kLW];
+  __shared__ __align__(16) float lpb[S::lrows * kLW];
+  __shared__ int item;
+  const BlockCols cols = block_cols<S>((int)g.D[2]);
+  zero_margin_rows<S, R>(lpa, lpb);
+  unsigned xcc;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+  int tried = 0;  // queues found empty (thread 0)
+  for (;;) {
+    __syncthreads();  // every wavefront is done with the previous item and its LDS reads
+    if (threadIdx.x == 0) {
+      int it = -1;
+      while (tried < kXcds) {
+        const int j = (int)(xcc + tried) % kXcds;
+        const int k = atomicAdd(&ctr[j], 1);
+        if (k < q.off[j + 1] - q.off[j]) {
+          it = q.off[j] + k;
+          break;
+        }
+        ++tried;
+      }
+      item = it;
+    }
+    __syncthreads();
+    const int it = item;
+    if (it < 0) break;  // uniform: every queue is empty
+    const int4 w = items[it];  // row strip, column chunk, first shot, end shot
+    Tile tl;
+    tl.ry = w.x;
+    tl.cx = w.y;
+    tl.grp = 0;
+    tl.valid = true;
+    k4_any_tile<R, ABS, TY, T>(g, io, shots, w.z, w.w, tl, cols, lpa, lpb);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ctr[kXcds], 1) == (int)gridDim.x - 1) {  // the last workgroup out
+      for (int j = 0; j <= kXcds; ++j) ctr[j] = 0;
+      __threadfence();
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2345,49 +2440,105 @@ bool fast2d_adj_block_supported(int ty, int depth) {
   return (ty == 8 || ty == 16) && depth >= 2 && depth <= kMaxDepth;
 }
 
-template <int R, int ABS, int TY, int T>
-static void launch_steps_t(const Geo<float>& g, const BlockIO& io, int shots, int spw, int xm,
-                           hipStream_t st) {
-  using S = BlockShape<TY, R, T>;
-  const int tiles =
-      block_tiles(block_cols<S>((int)g.D[2]), div_up((int)g.D[1], TY), div_up(shots, spw));
-  hipLaunchKernelGGL((k4_steps<R, ABS, TY, T>), dim3((unsigned)(kXcds * div_up(tiles, kXcds))),
-                     dim3(S::threads), 0, st, g, io, shots, spw, xm & 1);
-}
-
-template <int R, int ABS, int TY>
-static void steps_depth(const Geo<float>& g, const BlockIO& io, int shots, int depth, int spw,
-                        int xm, hipStream_t st) {
-  switch (depth) {
-    case 2: launch_steps_t<R, ABS, TY, 2>(g, io, shots, spw, xm, st); break;
-    case 3: launch_steps_t<R, ABS, TY, 3>(g, io, shots, spw, xm, st); break;
-    default: launch_steps_t<R, ABS, TY, 4>(g, io, shots, spw, xm, st); break;
-  }
-}
-
-template <int R, int ABS>
-static void steps_ty(const Geo<float>& g, const BlockIO& io, int shots, int ty, int depth,
-                     int spw, int xm, hipStream_t st) {
-  switch (ty) {
-    case 8: steps_depth<R, ABS, 8>(g, io, shots, depth, spw, xm, st); break;
-    case 32: steps_depth<R, ABS, 32>(g, io, shots, depth, spw, xm, st); break;
-    default: steps_depth<R, ABS, 16>(g, io, shots, depth, spw, xm, st); break;
+// Calls fn(R, ABS, TY, T) with the compile-time shape of a blocked forward launch (integral
+// constants), for the instances the library builds.
+template <class Fn>
+static void with_block_shape(int order, int absorbing, int ty, int depth, Fn&& fn) {
+  auto by_depth = [&](auto r, auto abs, auto rows) {
+    switch (depth) {
+      case 2: fn(r, abs, rows, std::integral_constant<int, 2>{}); break;
+      case 3: fn(r, abs, rows, std::integral_constant<int, 3>{}); break;
+      default: fn(r, abs, rows, std::integral_constant<int, 4>{}); break;
+    }
+  };
+  auto by_rows = [&](auto r, auto abs) {
+    switch (ty) {
+      case 8: by_depth(r, abs, std::integral_constant<int, 8>{}); break;
+      case 32: by_depth(r, abs, std::integral_constant<int, 32>{}); break;
+      default: by_depth(r, abs, std::integral_constant<int, 16>{}); break;
+    }
+  };
+  using R1 = std::integral_constant<int, 1>;
+  using R2 = std::integral_constant<int, 2>;
+  switch (absorbing * 2 + (order == 4 ? 1 : 0)) {
+    case NW_NONE * 2 + 0: by_rows(R1{}, std::integral_constant<int, NW_NONE>{}); break;
+    case NW_NONE * 2 + 1: by_rows(R2{}, std::integral_constant<int, NW_NONE>{}); break;
+    case NW_SPONGE * 2 + 0: by_rows(R1{}, std::integral_constant<int, NW_SPONGE>{}); break;
+    case NW_SPONGE * 2 + 1: by_rows(R2{}, std::integral_constant<int, NW_SPONGE>{}); break;
+    case NW_CERJAN * 2 + 0: by_rows(R1{}, std::integral_constant<int, NW_CERJAN>{}); break;
+    case NW_CERJAN * 2 + 1: by_rows(R2{}, std::integral_constant<int, NW_CERJAN>{}); break;
+    default: break;  // PML plans are never blocked (their face fields are not carried)
   }
 }
 
 void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& io, int shots,
-                  int ty, int depth, int spw, int xm, void* stream) {
+                  int ty, int depth, int spw, int xm, const BlockQueue* queue, void* stream) {
   hipStream_t st = (hipStream_t)stream;
-  const int key = absorbing * 2 + (order == 4 ? 1 : 0);
-  switch (key) {
-    case NW_NONE * 2 + 0: steps_ty<1, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
-    case NW_NONE * 2 + 1: steps_ty<2, NW_NONE>(g, io, shots, ty, depth, spw, xm, st); break;
-    case NW_SPONGE * 2 + 0: steps_ty<1, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
-    case NW_SPONGE * 2 + 1: steps_ty<2, NW_SPONGE>(g, io, shots, ty, depth, spw, xm, st); break;
-    case NW_CERJAN * 2 + 0: steps_ty<1, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
-    case NW_CERJAN * 2 + 1: steps_ty<2, NW_CERJAN>(g, io, shots, ty, depth, spw, xm, st); break;
-    default: break;  // PML plans are never blocked (their face fields are not carried)
-  }
+  with_block_shape(order, absorbing, ty, depth, [&](auto r, auto abs, auto rows, auto t) {
+    constexpr int R = decltype(r)::value, ABS = decltype(abs)::value;
+    constexpr int TY = decltype(rows)::value, T = decltype(t)::value;
+    using S = BlockShape<TY, R, T>;
+    if (queue) {
+      BlockQueues q;
+      for (int j = 0; j <= kXcds; ++j) q.off[j] = queue->off[j];
+      hipLaunchKernelGGL((k4_queue<R, ABS, TY, T>), dim3((unsigned)queue->grid),
+                         dim3(S::threads), 0, st, g, io, shots, queue->items, q, queue->ctr);
+    } else {
+      const int tiles =
+          block_tiles(block_cols<S>((int)g.D[2]), div_up((int)g.D[1], TY), div_up(shots, spw));
+      hipLaunchKernelGGL((k4_steps<R, ABS, TY, T>),
+                         dim3((unsigned)(kXcds * div_up(tiles, kXcds))), dim3(S::threads), 0,
+                         st, g, io, shots, spw, xm & 1);
+    }
+  });
+}
+
+int fast2d_steps_queue_items(const Geo<float>& g, int order, int absorbing, int ty, int depth,
+                             int shots, int spw, int split_tail, std::vector<int4>& items,
+                             int* off, int* per_cu) {
+  int rc = NW_ERR_ARG;
+  with_block_shape(order, absorbing, ty, depth, [&](auto r, auto abs, auto rows, auto t) {
+    constexpr int R = decltype(r)::value, ABS = decltype(abs)::value;
+    constexpr int TY = decltype(rows)::value, T = decltype(t)::value;
+    using S = BlockShape<TY, R, T>;
+    const BlockCols cols = block_cols<S>((int)g.D[2]);
+    const int nY = div_up((int)g.D[1], TY), nG = div_up(shots, spw);
+    const int per_grp = block_tiles(cols, nY, 1), tiles = per_grp * nG;
+    const int per = div_up(tiles, kXcds);
+    items.clear();
+    for (int j = 0; j < kXcds; ++j) {
+      off[j] = (int)items.size();
+      const int t0 = j * per, t1 = std::min(tiles, t0 + per), listed = t1 - t0;
+      int k = 0;
+      for (int tt = t0; tt < t1; ++tt, ++k) {
+        // tile tt in tile_of_cols' order (without the XCD map, which these ranges replace)
+        const int grp = tt / per_grp, u = tt - grp * per_grp;
+        int cx, ry;
+        if (!cols.split) {
+          cx = u % cols.full;
+          ry = u / cols.full;
+        } else {
+          const int pair = u / (2 * cols.full + 1), v = u - pair * (2 * cols.full + 1);
+          cx = v < 2 * cols.full ? v % cols.full : cols.full;
+          ry = v < 2 * cols.full ? 2 * pair + v / cols.full : 2 * pair;
+        }
+        if (ry >= nY) continue;  // the missing second strip of an odd last pair
+        const int s0 = grp * spw, s1 = std::min(shots, s0 + spw);
+        if (k >= listed - split_tail) {
+          for (int s = s0; s < s1; ++s) items.push_back(make_int4(ry, cx, s, s + 1));
+        } else {
+          items.push_back(make_int4(ry, cx, s0, s1));
+        }
+      }
+    }
+    off[kXcds] = (int)items.size();
+    int n = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, reinterpret_cast<const void*>(&k4_queue<R, ABS, TY, T>), S::threads, 0);
+    *per_cu = e == hipSuccess ? std::max(n, 1) : 1;
+    rc = e == hipSuccess ? NW_OK : NW_ERR_HIP;
+  });
+  return rc;
 }
 
 template <int R, int ABS, int TY, int T>

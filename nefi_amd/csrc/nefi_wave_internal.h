@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace nw {
 
@@ -166,8 +167,25 @@ void fast2d_adjoint(const Geo<float>& g, int order, int absorbing, const AdjIO<f
                     int shots, int ty, int spw, int xcd_map, void* stream);
 // depth 2..kMaxDepth, ty 8/16/32; returns false for a shape it has no kernel for
 bool fast2d_block_supported(int ty, int depth);
+// Work queue of one plan's blocked forward launches (k4_queue): items (row strip, column
+// chunk, first shot, end shot) of queue j in [off[j], off[j + 1]), the queues' counters
+// (kXcds + 1 ints, zero between launches) and the workgroups of a launch.
+constexpr int kQueues = 8;
+struct BlockQueue {
+  const int4* items;
+  int off[kQueues + 1];
+  int* ctr;
+  int grid;
+};
+// queue == nullptr: one workgroup per tile (k4_steps); else the work queue
 void fast2d_steps(const Geo<float>& g, int order, int absorbing, const BlockIO& io, int shots,
-                  int ty, int depth, int spw, int xcd_map, void* stream);
+                  int ty, int depth, int spw, int xcd_map, const BlockQueue* queue, void* stream);
+// The queue items of a plan's blocked launches (tiles, depth, shots per workgroup), the last
+// split_tail tiles of every queue split into single shots, and the workgroups of the queue
+// kernel that fit on one CU.
+int fast2d_steps_queue_items(const Geo<float>& g, int order, int absorbing, int ty, int depth,
+                             int shots, int spw, int split_tail, std::vector<int4>& items,
+                             int* off, int* per_cu);
 // traces of levels 1..levels-1 from a BlockIO tap buffer; j[k-1] < 0 skips level k
 struct LevelRecords {
   int j[kMaxDepth];

@@ -504,6 +504,11 @@ class _LeapfrogBase(Operator):
             # each nw_forward / nw_adjoint call becomes one hipGraph launch after its first use
             if os.environ.get("NEFI_AMD_GRAPH", "1") != "0":
                 prop.plan.set_option(nat.OPT_GRAPH, 1)
+            # scheduling of blocked forward launches: 1 work queue for calls without keep
+            # buffers (library default), 2 for all of them, 0 one workgroup per tile
+            queue = os.environ.get("NEFI_AMD_WORK_QUEUE")
+            if queue is not None and prop.plan.option(nat.OPT_WORK_QUEUE) != int(queue):
+                prop.plan.set_option(nat.OPT_WORK_QUEUE, int(queue))
             self._cache[full] = prop
         prop.grad_mode = self.grad_mode  # instances switch it after construction
         prop.checkpoint_every = self.checkpoint_every

@@ -40,7 +40,7 @@ def _survey(absorbing="sponge", order=4, n=(70, 300), shots=5, n_t=60):
     return Domain(n, ext), geo, c, kw
 
 
-def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1, keep_lap=1, depth_adj=1):
+def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1, keep_lap=1, depth_adj=1, queue=1):
     from nefi_amd import _native as nat
 
     x = torch.tensor(c, dtype=torch.float32, device=DEV, requires_grad=True)
@@ -56,6 +56,7 @@ def _grad(op, c, w, tiles=None, lds=(0, 0), xcd=1, depth=1, keep_lap=1, depth_ad
         prop.plan.set_option(nat.OPT_TIME_BLOCK, depth)
         prop.plan.set_option(nat.OPT_KEEP_LAP, keep_lap)
         prop.plan.set_option(nat.OPT_TIME_BLOCK_ADJOINT, depth_adj)
+        prop.plan.set_option(nat.OPT_WORK_QUEUE, queue)
         y = op({"c": x})
     (g,) = torch.autograd.grad((y * w).sum(), x)
     return y.detach().cpu().double().numpy(), g.cpu().double().numpy()
@@ -110,6 +111,40 @@ def test_time_blocking_is_bit_identical(absorbing, order):
         print(f"{absorbing} o{order} rows {ty} spw {spw} depth {depth} xcd {xcd}: "
               f"{blocked} blocked steps, traces {ey:.1e}, gradient {eg:.1e}", flush=True)
         assert blocked > 0
+        assert np.array_equal(y, y0) and np.array_equal(g, g0)
+
+
+@pytest.mark.parametrize("absorbing,order,n", [("sponge", 4, (70, 600)), ("cerjan", 4, (90, 84)),
+                                               ("none", 2, (54, 612)), ("sponge", 4, (70, 328)),
+                                               ("sponge", 2, (38, 200))])
+def test_work_queue_is_bit_identical(absorbing, order, n):
+    """Blocked forward launches as a work queue (persistent workgroups taking tiles, and single
+    shots of the last tiles, from per-XCD queues) give exactly the traces and gradient of one
+    workgroup per tile: quiet, boundary and split-column tiles, shot groups that do not divide
+    the shots, every blocked tile height and depth. Each gradient runs several queue launches
+    back to back, so the counters' reset between launches is exercised too."""
+    from nefi_amd import _native as nat
+
+    dom, geo, c, kw = _survey(absorbing, order, n=n, shots=7)
+    print(f"[case work queue {absorbing} o{order} {n}]", flush=True)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, dtype=torch.float32,
+                    generator=torch.Generator(DEV).manual_seed(31))
+    for ty, spw, depth in ((32, 4, 4), (16, 3, 3), (8, 2, 2), (32, 7, 2), (16, 1, 4)):
+        tiles = [(ty, spw), (16, 2)]
+        y0, g0 = _grad(op, c, w, tiles=tiles, lds=(1, 1), depth=depth, queue=0)
+        prop = next(iter(op._cache.values()))
+        assert prop.plan.option(nat.OPT_WORK_QUEUE_ITEMS) == 0
+        nat.profile_enable(True)
+        y, g = _grad(op, c, w, tiles=tiles, lds=(1, 1), depth=depth, queue=2)  # keep sweeps too
+        blocked = nat.profile_units(nat.K_STEPS)
+        nat.profile_enable(False)
+        items = prop.plan.option(nat.OPT_WORK_QUEUE_ITEMS)
+        print(f"queue {absorbing} o{order} {n} rows {ty} spw {spw} depth {depth}: {items} items, "
+              f"{blocked} blocked steps, traces {rel(y, y0):.1e}, gradient {rel(g, g0):.1e}",
+              flush=True)
+        assert items > 0 and blocked > 0
         assert np.array_equal(y, y0) and np.array_equal(g, g0)
 
 
