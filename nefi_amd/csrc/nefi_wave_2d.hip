@@ -1207,7 +1207,8 @@ __global__ void __launch_bounds__(BlockWaves<TY>::threads)
     if (threadIdx.x == 0) {
       int it = -1;
       while (tried < kXcds) {
-        const int j = (int)(xcc + tried) % kXcds;
+        // unsigned modulo: j stays in [0, kXcds) whatever the register's upper bits hold
+        const int j = (int)((xcc + (unsigned)tried) % (unsigned)kXcds);
         const int k = atomicAdd(&ctr[j], 1);
         if (k < q.off[j + 1] - q.off[j]) {
           it = q.off[j] + k;
