@@ -558,6 +558,7 @@ struct nw_plan {
   int tblock;
   float* tb_state[2];
   float* tapbuf;
+  int2* tap_rows;  // per shot: first and last grid row of its receiver taps (blocked sweeps)
   int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
   int fast3d;        // NW_OPT_FAST_3D: the 2.5-D tiled 3-D float32 kernels (when they cover it)
   int fast64;        // NW_OPT_FAST_F64: the LDS-tiled 2-D float64 forward step
@@ -892,6 +893,7 @@ static int forward_blocked(nw_plan* P, const float* m, void** state, int n_begin
       io.rec_first = d.rec_first;
       io.rec_count = d.rec_count;
       io.rec_idx = d.rec_idx;
+      io.tap_rows = reinterpret_cast<const int32_t*>(P->tap_rows);
     }
     const int pid = prof_begin(NW_K_STEPS, st, L);
     // keep sweeps run beside reverse steps in an overlapped reverse sweep, where persistent
@@ -1417,6 +1419,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tblock = 1;
   p->tb_state[0] = p->tb_state[1] = nullptr;
   p->tapbuf = nullptr;
+  p->tap_rows = nullptr;
   p->keep_lap_opt = -1;
   p->fast3d = fast3d_supported(p->gf, d.absorbing, d.dtype) ? 1 : 0;
   p->fast64 = fast2d64_supported(p->gd, d.absorbing, d.dtype) ? 1 : 0;
@@ -1454,6 +1457,7 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
   for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
     if (b) (void)hipFree(b);
+  if (plan->tap_rows) (void)hipFree(plan->tap_rows);
   if (plan->wq_items) (void)hipFree(plan->wq_items);
   if (plan->wq.ctr) (void)hipFree(plan->wq.ctr);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)
@@ -1468,6 +1472,36 @@ int32_t nw_plan_destroy(nw_plan* plan) {
 int64_t nw_grad_slices(const nw_plan* P) {
   if (P->d.dtype == NW_F32 && P->rows[1] > 0) return (P->d.n_shots + P->spw[1] - 1) / P->spw[1];
   return P->d.n_shots;
+}
+
+// First and last grid row of every shot's receiver taps (an empty range for a shot without
+// receivers): a blocked forward tile scans a shot's taps only when they reach its output rows.
+// Reads the receiver tables back once, when the blocking buffers are allocated.
+static int build_tap_rows(nw_plan* P) {
+  const nw_desc& d = P->d;
+  const int S = d.n_shots;
+  std::vector<int2> tr(S, make_int2(INT32_MAX, -1));
+  if (d.rec_max > 0) {
+    std::vector<int32_t> first(S), count(S);
+    NW_HIP(hipMemcpy(first.data(), d.rec_first, S * sizeof(int32_t), hipMemcpyDefault));
+    NW_HIP(hipMemcpy(count.data(), d.rec_count, S * sizeof(int32_t), hipMemcpyDefault));
+    int64_t rows = 0;
+    for (int s = 0; s < S; ++s) rows = std::max<int64_t>(rows, (int64_t)first[s] + count[s]);
+    std::vector<int32_t> idx((size_t)rows * d.rec_taps);
+    if (rows > 0)
+      NW_HIP(hipMemcpy(idx.data(), d.rec_idx, idx.size() * sizeof(int32_t), hipMemcpyDefault));
+    const int64_t P2 = P->gf.P2;
+    for (int s = 0; s < S; ++s)
+      for (int64_t e = (int64_t)first[s] * d.rec_taps; e < ((int64_t)first[s] + count[s]) * d.rec_taps;
+           ++e) {
+        const int y = (int)(idx[e] / P2);
+        tr[s].x = std::min(tr[s].x, y);
+        tr[s].y = std::max(tr[s].y, y);
+      }
+  }
+  NW_HIP(hipMalloc((void**)&P->tap_rows, S * sizeof(int2)));
+  NW_HIP(hipMemcpy(P->tap_rows, tr.data(), S * sizeof(int2), hipMemcpyHostToDevice));
+  return NW_OK;
 }
 
 // (Re)builds the work queue of the blocked forward launches for the plan's current forward
@@ -1594,6 +1628,7 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
         NW_HIP(hipMalloc((void**)&P->tb_state[k], (size_t)P->shot_bytes));
       NW_HIP(hipMalloc((void**)&P->tapbuf, (size_t)taps * sizeof(float)));
       NW_HIP(hipMalloc((void**)&P->injv, (size_t)inj * sizeof(float)));
+      NW_TRY(build_tap_rows(P));
     }
     (option == NW_OPT_TIME_BLOCK ? P->tblock : P->tblock_adj) = value;
     return option == NW_OPT_TIME_BLOCK ? build_queue(P) : NW_OK;

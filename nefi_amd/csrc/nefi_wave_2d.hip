@@ -877,6 +877,24 @@ __device__ __forceinline__ int thread_index() {
   return t;
 }
 
+// Issue the loads of a wavefront's rows c0 .. c0 + mr - 1 of one region of a quiet tile (grid
+// rows y .. y + mr - 1, all inside the grid), each lane its own four columns x .. x + 3, straight
+// into LDS (global_load_lds: lane l lands 16 bytes after lane l - 1, from the wave-uniform
+// address of the row's first output column). The region's margin columns are not loaded: they
+// only reach cells within R T of the region's edge. Completion is awaited with s_waitcnt.
+template <typename S, int R>
+__device__ __forceinline__ void rows_async(const float* P, int y, int x, int P2, int c0,
+                                           float* buf) {
+#pragma unroll
+  for (int k = 0; k < S::mr; ++k) {
+    const float* src = P + ((y + k) * P2 + x);
+    float* dst = buf + (R + c0 + k) * kLW + 4;
+    __builtin_amdgcn_global_load_lds((gmem_void*)src, (lds_void*)dst, 16, 0, 0);
+  }
+}
+
+// The same for the whole region, margins included, with the workgroup's threads spread over
+// its float4 elements (blocked reverse steps).
 template <typename S, int R>
 __device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, int P2,
                                              float* buf) {
@@ -949,7 +967,7 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   };
   float4 pvn[Q ? S::mr : 1];
   if constexpr (Q) {
-    region_async<S, R>(io.p + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
+    rows_async<S, R>(io.p + (int64_t)s_first * g.cells, ry0 + c0, x, P2, c0, lpa);
     load_prev(s_first, pvn);
   }
   // one shot; B selects the LDS buffer at compile time (even shots A, odd shots B), so the
@@ -965,27 +983,19 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
 #pragma unroll
       for (int k = 0; k < S::mr; ++k) pv[k] = pvn[k];
     } else {
-      float4 v[S::per_thread];
+      // the wavefront's own rows of p^n, each lane its four columns (zero outside the grid);
+      // they are also the lane's current level
       const float* P = io.p + base;
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) {
-        const int i = thread_index() + S::threads * k;
-        const int row = i / (kLW / 4), c4 = i % (kLW / 4);
-        const bool lower = SP && c4 >= kLW / 8;  // the lower tile's half of the LDS row
-        const int gy = (lower ? ryA + TY : ryA) + row;
-        const int gx = lower ? rx0 + 4 * (c4 - kLW / 8) : rx0 - 4 + 4 * c4;
-        const bool in = (S::threads * (k + 1) <= S::n4 || i < S::n4);
-        const bool ok = in && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
-        v[k] = ok ? ld4(P + (gy * P2 + gx)) : f4(0.f);
+      for (int k = 0; k < S::mr; ++k) {
+        const int y = ry0 + c0 + k;
+        cur[k] = (inx && y >= 0 && y < D1) ? ld4(P + (y * P2 + x)) : f4(0.f);
       }
       load_prev(s, pv);
       __syncthreads();  // the previous shot's reads of lp are done
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) {
-        const int i = thread_index() + S::threads * k;
-        if (S::threads * (k + 1) <= S::n4 || i < S::n4)
-          *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
-      }
+      for (int k = 0; k < S::mr; ++k)
+        *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = cur[k];
     }
     // does a source tap of this shot fall in this wavefront's rows of the region? (uniform)
     bool src_rows = false;
@@ -996,7 +1006,12 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     }
     // receiver taps of this shot inside the output tile: LDS offset, or -1
     int toff[BlockWaves<TY>::tap_chunks];
-    const int n_tap = io.tapbuf ? io.rec_count[s] * io.taps : 0;
+    int n_tap = io.tapbuf ? io.rec_count[s] * io.taps : 0;
+    if (n_tap > 0) {
+      // no scan when the shot's taps all lie outside the tile's output rows (uniform)
+      const int o0 = ryA + S::hy, o1 = o0 + (SP ? 2 * TY : TY);
+      if (ldc(io.tap_rows, 2 * s + 1) < o0 || ldc(io.tap_rows, 2 * s) >= o1) n_tap = 0;
+    }
     const int tap0 = io.tapbuf ? io.rec_first[s] * io.taps : 0;
 #pragma unroll
     for (int c = 0; c < BlockWaves<TY>::tap_chunks; ++c) {
@@ -1022,12 +1037,12 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
     if constexpr (!Q) __syncthreads();
     if constexpr (Q) {
       if (s + 1 < s_end) {  // stream the next shot in while this one computes
-        region_async<S, R>(io.p + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
+        rows_async<S, R>(io.p + base + g.cells, ry0 + c0, x, P2, c0, B ? lpa : lpb);
         load_prev(s + 1, pvn);
       }
-    }
 #pragma unroll
-    for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+      for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+    }
     if (io.ck_prev) {
 #pragma unroll
       for (int k = 0; k < S::mr; ++k) {
@@ -1126,8 +1141,9 @@ __device__ __forceinline__ void k4_tile(const Geo<float>& g, const BlockIO& io, 
   }
 }
 
-// The R zero rows above and below the region in both LDS buffers (the stencil of the region's
-// first and last rows reads them); tiles never write them.
+// The R zero rows above and below the region and the 4 margin columns on each side of every
+// row in both LDS buffers (the stencil of the region's edge rows and columns reads them); tiles
+// never write them.
 template <class S, int R>
 __device__ __forceinline__ void zero_margin_rows(float* lpa, float* lpb) {
   for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
@@ -1135,6 +1151,12 @@ __device__ __forceinline__ void zero_margin_rows(float* lpa, float* lpb) {
     lpa[(S::lrows - R) * kLW + i] = 0.f;
     lpb[i] = 0.f;
     lpb[(S::lrows - R) * kLW + i] = 0.f;
+  }
+  for (int r = threadIdx.x; r < S::lrows; r += S::threads) {
+    st4(lpa + r * kLW, f4(0.f));
+    st4(lpa + r * kLW + kLW - 4, f4(0.f));
+    st4(lpb + r * kLW, f4(0.f));
+    st4(lpb + r * kLW + kLW - 4, f4(0.f));
   }
 }
 

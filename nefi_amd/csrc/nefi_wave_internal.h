@@ -135,6 +135,7 @@ struct BlockIO {
   const int32_t* rec_first;
   const int32_t* rec_count;
   const int32_t* rec_idx;
+  const int32_t* tap_rows;   // shot s: first (2s) and last (2s + 1) grid row of its taps
 };
 
 // Temporally blocked reverse steps (2-D float32): levels k = 1..levels compute lambda^{nh-k}
