@@ -16,8 +16,11 @@
  *    Wavefield rows are `pitch` elements apart (see nw_desc.pitch); the sound speed c,
  *    initial pressure p0 and the outputs on the field grid are dense.
  *  - A wavefield buffer holds `n_shots` consecutive grids. PML auxiliary fields are face
- *    grids (extent + 1 along their own axis), also `n_shots` consecutive; a face buffer is
- *    rounded up to a multiple of 64 elements, so every slot offset stays 256-byte aligned.
+ *    grids (extent + 1 along their own axis), also `n_shots` consecutive; rows of the axis 2
+ *    face grid are stored at a pitch (a multiple of 4 elements, at least dims[2] + 1 and the
+ *    wavefield pitch) so that they start 16-byte aligned, and a face buffer is rounded up to
+ *    a multiple of 64 elements, so every slot offset stays 256-byte aligned. nw_face_layout
+ *    reports both.
  *  - `dtype` selects float (0) or double (1) for every real-valued buffer of a plan.
  *  - Device buffers (wavefields, face fields, model, traces, slots) must be 16-byte aligned:
  *    the tiled kernels move 16-byte vectors. Allocations from hipMalloc or PyTorch are.
@@ -35,7 +38,7 @@
 extern "C" {
 #endif
 
-#define NW_ABI_VERSION 3
+#define NW_ABI_VERSION 4
 
 enum nw_status {
   NW_OK = 0,
@@ -255,6 +258,16 @@ int64_t nw_grad_slices(const nw_plan* plan);
  * options: slots of kept Laplacians hold one wavefield buffer, PML face fields included in it). */
 int64_t nw_ckpt_slot_bytes(const nw_plan* plan);
 int64_t nw_keep_slot_bytes(const nw_plan* plan);
+
+/* Layout of the plan's PML face buffer along grid axis `axis` (0..2): *elems = elements of one
+ * buffer (all shots, rounded up to a multiple of 64; 0 when the plan has no faces along the
+ * axis), *row_pitch = elements between consecutive rows of one shot's face grid (dims[2] for
+ * the faces of axes 0 and 1; for axis 2 a multiple of 4, >= dims[2] + 1 and >= the wavefield
+ * pitch; the elements past face dims[2] of a row are never read or written). The face grid of
+ * axis 2 for shot s starts at s * dims[0] * dims[1] * row_pitch. New in ABI version 4 (which
+ * pitched the axis 2 face rows); the reference has no counterpart: nefi keeps its face fields
+ * as PyTorch tensors (nefi/physics/wave.py:529-541). */
+int32_t nw_face_layout(const nw_plan* plan, int32_t axis, int64_t* elems, int64_t* row_pitch);
 
 /* Reverse sweep over steps n_end-1 down to n_begin: the discrete adjoint of the leapfrog that
  * autograd derives from wave.py:520-554 and replays through timestep.py:151-169.

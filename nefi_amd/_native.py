@@ -23,7 +23,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libnefi_wave.so"
 NW_NONE, NW_SPONGE, NW_CERJAN, NW_PML = 0, 1, 2, 3
 NW_F32, NW_F64 = 0, 1
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # every symbol include/nefi_wave.h declares
 EXPORTS = (
@@ -36,6 +36,7 @@ EXPORTS = (
     "nw_forward",
     "nw_ckpt_slot_bytes",
     "nw_keep_slot_bytes",
+    "nw_face_layout",
     "nw_adjoint",
     "nw_model_grad",
     "nw_ic_state",
@@ -146,6 +147,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nw_forward": (i32, [vp, vp, vpp, vpp, i32, i32, vp, vp, i32, vp, i32, i32, vp]),
         "nw_ckpt_slot_bytes": (i64, [vp]),
         "nw_keep_slot_bytes": (i64, [vp]),
+        "nw_face_layout": (i32, [vp, i32, POINTER(c_int64), POINTER(c_int64)]),
         "nw_adjoint": (i32, [vp, vp, vp, vpp, vpp, i32, i32, vp, i32, vp, i32, vp, i32, vp]),
         "nw_model_grad": (i32, [vp, vp, vp, vp, vp]),
         "nw_ic_state": (i32, [vp, vp, vp, vp, vp, vp]),

@@ -101,10 +101,11 @@ __device__ __forceinline__ int64_t stride_of(const Geo<T>& g, int a) {
   return a == 0 ? g.D[1] * g.P2 : (a == 1 ? g.P2 : 1);
 }
 
-// flat index of face f along axis a at the other coordinates of `co`
+// flat index of face f along axis a at the other coordinates of `co` (rows of the axis 2 face
+// grid are FP2 elements apart, so they start 16-byte aligned like the wavefield rows)
 template <typename T>
 __device__ __forceinline__ int64_t face_at(const Geo<T>& g, int a, const int64_t* co, int64_t f) {
-  const int64_t d1 = g.D[1] + (a == 1), d2 = g.D[2] + (a == 2);
+  const int64_t d1 = g.D[1] + (a == 1), d2 = a == 2 ? g.FP2 : g.D[2];
   const int64_t c0 = a == 0 ? f : co[0], c1 = a == 1 ? f : co[1], c2 = a == 2 ? f : co[2];
   return (c0 * d1 + c1) * d2 + c2;
 }
@@ -506,10 +507,11 @@ Geo<T> make_geo(const nw_desc& d) {
     g.quiet[a][0] = 0;
     g.quiet[a][1] = (int)g.D[a];
   }
+  g.FP2 = (std::max<int64_t>(g.P2, g.D[2] + 1) + 3) / 4 * 4;
   for (int a = 0; a < 3; ++a) {
     int64_t f = 1;
-    for (int b = 0; b < 3; ++b) f *= g.D[b] + (b == a);
-    g.fcells[a] = f;
+    for (int b = 0; b < 2; ++b) f *= g.D[b] + (b == a);
+    g.fcells[a] = f * (a == 2 ? g.FP2 : g.D[2]);
   }
   // coefficients of nefi/physics/wave.py:121 per axis, summed into the centre in f64
   const double c0 = d.order == 4 ? -2.5 : -2.0;
@@ -1696,6 +1698,15 @@ int32_t nw_plan_get_option(const nw_plan* P, int32_t option, int32_t* value) {
 
 int64_t nw_ckpt_slot_bytes(const nw_plan* P) {
   return 2 * P->shot_bytes + P->face_bytes[0] + P->face_bytes[1] + P->face_bytes[2];
+}
+
+int32_t nw_face_layout(const nw_plan* P, int32_t axis, int64_t* elems, int64_t* row_pitch) {
+  if (!P || axis < 0 || axis > 2 || !elems || !row_pitch)
+    return fail(NW_ERR_ARG, "nw_face_layout: bad plan, axis or output pointer");
+  const int64_t es = P->d.dtype == NW_F64 ? 8 : 4;
+  *elems = P->face_bytes[axis] / es;
+  *row_pitch = axis == 2 ? P->gf.FP2 : P->gf.D[2];
+  return NW_OK;
 }
 
 int64_t nw_keep_slot_bytes(const nw_plan* P) {

@@ -1534,10 +1534,11 @@ __global__ void __launch_bounds__(256) k_sample_levels(Geo<float> g, SampleIO<fl
 // PML (2-D): LDS-tiled forward and reverse steps with the face fields
 // ------------------------------------------------------------------------------------------
 // Face fields psi_a live on the faces normal to axis a: psi_y rows of D2 faces (D1 + 1 rows),
-// psi_x rows of D2 + 1 faces (D1 rows), unpitched. A lane's four cells x..x+3 of row y touch
-// x-faces x..x+4 (face x+4 is the next lane's face x: it computes it again, with the same
-// operations) and y-faces of rows y and y+1 (the next row of the wavefront computes row y+1
-// again). Each face is written by the cell on its high side; the last cell of a row (column)
+// psi_x rows of D2 + 1 faces (D1 rows) stored FP2 apart (a multiple of 4), so a lane's faces
+// x..x+3 are one aligned float4. A lane's four cells x..x+3 of row y touch x-faces x..x+4 (face
+// x+4 is the next lane's face x: it takes it from that lane with a shuffle and computes it
+// again, with the same operations; lane 63 and the grid's last lane load it) and y-faces of
+// rows y and y+1 (the next row of the wavefront computes row y+1 again). Each face is written by the cell on its high side; the last cell of a row (column)
 // also writes the final face. Per face f between cells f-1 and f:
 //   forward:  psi'_f = alpha_f psi_f + (zeta_other - zf_f) kappa_f (p_f - p_{f-1})
 //             lap += ((psi'_{f+1} + psi_{f+1}) - (psi'_f + psi_f)) / (2h)   (per cell)
@@ -1588,6 +1589,18 @@ __device__ __forceinline__ void tile_store_w(float* lds, const float4* v) {
   }
 }
 
+// The x-faces x..x+4 of a lane from its aligned float4 of faces x..x+3 (lanes whose four
+// cells lie in the quiet box pass zeros, the faces' values there) and face x+4 taken from the
+// next lane, or from `edge` (loaded by lane 63 and the grid's last lane).
+__device__ __forceinline__ void faces5(const float4& v, float edge, bool own_edge, float* f) {
+  const float nx = __shfl_down(v.x, 1, 64);
+  f[0] = v.x;
+  f[1] = v.y;
+  f[2] = v.z;
+  f[3] = v.w;
+  f[4] = own_edge ? edge : nx;
+}
+
 // One tile of k3_step_pml. Q: the tile lies in the PML's quiet box, where every face-field
 // coefficient and the damping vanish: it neither reads nor writes face fields (they are zero
 // there and stay zero; face buffers, snapshots and keep slots hold zeros in the box from
@@ -1597,13 +1610,15 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
                                                  int shots, int spw, const Tile& tl, float* lp) {
   using S = TileShape<TY, R>;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int F2 = D2 + 1;
+  const int F2 = (int)g.FP2;
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * S::my_rows;
   const bool act = x < D2;
   const bool last_col = x + 4 == D2;
+  // lanes that load their face x+4 (the next lane's face x lies in another tile or past D2)
+  const bool own_edge = lane == 63 || last_col;
   const float4 hx = act ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<NW_PML> dcol = damp4_col<NW_PML>(hx);
   const float4 zcx = act ? ld4(g.zc[2] + x) : f4(0.f);
@@ -1671,29 +1686,27 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
           // x faces x..x+4 of row y
           const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
           float ox[5], nx[5];
+          faces5(lq ? f4(0.f) : ld4(ox_row), (!lq && own_edge && x + 4 <= D2) ? ox_row[4] : 0.f,
+                 own_edge, ox);
           const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
 #pragma unroll
-          for (int j = 0; j < 5; ++j) {
-            ox[j] = (!lq && (j < 4 || x + 4 <= D2)) ? ox_row[j] : 0.f;
+          for (int j = 0; j < 5; ++j)
             nx[j] =
                 __builtin_fmaf((zcy - px.zf[j]) * px.k[j], pf[j + 1] - pf[j], px.a[j] * ox[j]);
-          }
           lap.x = lap.x + ((nx[1] + ox[1]) - (nx[0] + ox[0])) * ih2;
           lap.y = lap.y + ((nx[2] + ox[2]) - (nx[1] + ox[1])) * ih2;
           lap.z = lap.z + ((nx[3] + ox[3]) - (nx[2] + ox[2])) * ih2;
           lap.w = lap.w + ((nx[4] + ox[4]) - (nx[3] + ox[3])) * ih2;
           if (!lq) {
             float* nx_row = io.psi_new[2] + fx + (int64_t)y * F2 + x;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) nx_row[j] = nx[j];
+            st4(nx_row, make_float4(nx[0], nx[1], nx[2], nx[3]));
             if (last_col) nx_row[4] = nx[4];
           }
           if (io.ck_psi[1] && !lq) {
             st4(io.ck_psi[1] + fy + ((int64_t)y * D2 + x), oy0);
             if (y == D1 - 1) st4(io.ck_psi[1] + fy + ((int64_t)(y + 1) * D2 + x), oy1);
             float* cx = io.ck_psi[2] + fx + (int64_t)y * F2 + x;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) cx[j] = ox[j];
+            st4(cx, make_float4(ox[0], ox[1], ox[2], ox[3]));
             if (last_col) cx[4] = ox[4];
           }
         }
@@ -1772,13 +1785,14 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
   using S = AdjShape<TY, R>;
   constexpr int MR = S::my_rows;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
-  const int F2 = D2 + 1;
+  const int F2 = (int)g.FP2;
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int x0 = tl.cx * 256, y0 = tl.ry * TY;
   const int x = x0 + 4 * lane;
   const int r0 = wave * MR;
   const bool act = x < D2;
   const bool last_col = x + 4 == D2;
+  const bool own_edge = lane == 63 || last_col;  // loads its face x+4 (see faces5)
   const float4 hx = act ? ld4(g.damp[2] + x) : f4(0.f);
   const Damp4<NW_PML> dcol = damp4_col<NW_PML>(hx);
   const float4 zcx = act ? ld4(g.zc[2] + x) : f4(0.f);
@@ -1806,8 +1820,8 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
     tile_load_w<S, R>(L1, y0, x0, D1, D2, P2, vl);
     if (p_tile) tile_load_w<S, R>(io.p_n + base, y0, x0, D1, D2, P2, vp);
     // this wave's rows: lambda^{n+1}, lambda^{n+2}, the kept Laplacian and the adjoint faces
-    float4 lc[MR], l2c[MR], lpn[MR], i0[MR], i1[MR];
-    float inx[MR][5];
+    float4 lc[MR], l2c[MR], lpn[MR], i0[MR], i1[MR], ix[MR];
+    float ixe[MR];
 #pragma unroll
     for (int k = 0; k < MR; ++k) {
       const int y = y0 + r0 + k;
@@ -1818,9 +1832,10 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
       const bool faces = !Q && ok && !(col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1]);
       i0[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x)) : f4(0.f);
       i1[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x)) : f4(0.f);
+      // x faces x..x+3, and x+4 where the next lane does not hold it
       const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
-#pragma unroll
-      for (int j = 0; j < 5; ++j) inx[k][j] = (faces && (j < 4 || x + 4 <= D2)) ? in_row[j] : 0.f;
+      ix[k] = faces ? ld4(in_row) : f4(0.f);
+      ixe[k] = (faces && own_edge && x + 4 <= D2) ? in_row[4] : 0.f;
     }
     lds_barrier();  // the previous shot's reads of lq / lp are done
 #pragma unroll
@@ -1904,15 +1919,18 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
               lapp.w = lapp.w + ((h1.w + w1.w) - (h0.w + w0.w)) * ih1;
             }
           }
-          // x faces x..x+4 of row y
+          // x faces x..x+4 of row y. Face x+4 of a lane whose right neighbour skipped its
+          // faces (quiet) reads as zero instead of its stored value: its coefficient b[4] is
+          // zero there, so only the sign of a zero product can differ.
           {
             const float4 qc = qw[R];
             const float qf[6] = {qn.y, qc.x, qc.y, qc.z, qc.w, qn.z};  // q at x-1 .. x+4
-            float t[5], dq[5], b[5];
+            float inx[5], t[5], dq[5], b[5];
+            faces5(ix[k], ixe[k], own_edge, inx);
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
               dq[j] = qf[j + 1] - qf[j];
-              t[j] = inx[k][j] - ih2 * dq[j];
+              t[j] = inx[j] - ih2 * dq[j];
               b[j] = (zcy - px.zf[j]) * px.k[j];
             }
             lam.x = lam.x - (b[1] * t[1] - b[0] * t[0]);
@@ -1921,20 +1939,27 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
             lam.w = lam.w - (b[4] * t[4] - b[3] * t[3]);
             float* out_row = io.psib_out[2] + fx + (int64_t)y * F2 + x;
             if (!lqr) {
-#pragma unroll
-              for (int j = 0; j < 4; ++j) out_row[j] = px.a[j] * t[j] - ih2 * dq[j];
+              st4(out_row, make_float4(px.a[0] * t[0] - ih2 * dq[0], px.a[1] * t[1] - ih2 * dq[1],
+                                       px.a[2] * t[2] - ih2 * dq[2],
+                                       px.a[3] * t[3] - ih2 * dq[3]));
               if (last_col) out_row[4] = px.a[4] * t[4] - ih2 * dq[4];
             }
-            if (p_tile && !lqr) {
+            if (p_tile) {
+              // psi^{n+1/2} + psi^{n-1/2} at faces x..x+4 (quiet lanes pass zeros, the forward
+              // face fields' values in the box, to the shuffle and add nothing themselves)
               const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
               const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
+              const float4 h4 = lqr ? f4(0.f) : ld4(hr), w4 = lqr ? f4(0.f) : ld4(wr);
+              const bool e = !lqr && own_edge && x + 4 <= D2;
               float hw[5];
-#pragma unroll
-              for (int j = 0; j < 5; ++j) hw[j] = hr[j] + wr[j];
-              lapp.x = lapp.x + (hw[1] - hw[0]) * ih2;
-              lapp.y = lapp.y + (hw[2] - hw[1]) * ih2;
-              lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
-              lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
+              faces5(make_float4(h4.x + w4.x, h4.y + w4.y, h4.z + w4.z, h4.w + w4.w),
+                     e ? hr[4] + wr[4] : 0.f, own_edge, hw);
+              if (!lqr) {
+                lapp.x = lapp.x + (hw[1] - hw[0]) * ih2;
+                lapp.y = lapp.y + (hw[2] - hw[1]) * ih2;
+                lapp.z = lapp.z + (hw[3] - hw[2]) * ih2;
+                lapp.w = lapp.w + (hw[4] - hw[3]) * ih2;
+              }
             }
           }
         }

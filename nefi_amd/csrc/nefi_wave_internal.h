@@ -23,7 +23,8 @@ struct Geo {
   int64_t nabs[3];    // absorbing cells per side
   int64_t P2;         // row pitch of axis 2 (>= D[2])
   int64_t cells;      // D0*D1*P2: elements of one shot's wavefield
-  int64_t fcells[3];  // cells of the face grid of each axis
+  int64_t fcells[3];  // cells of the face grid of each axis (rows of axis 2 faces pitched)
+  int64_t FP2;        // row pitch of the axis 2 face grid: a multiple of 4, >= D[2] + 1, P2
   int used[3];        // axis carries a stencil
   int nd;             // physical dimensions
   T cc;               // centre coefficient: sum over used axes of c0 / h^2

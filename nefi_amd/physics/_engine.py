@@ -188,12 +188,15 @@ class DevicePlan:
         self.ckpt_slot = lib.nw_ckpt_slot_bytes(handle) // self.esize
         S, cells = host.n_shots, host.cells
         self.shot_numel = S * cells
-        self.face_numel = []
+        # PML face buffers as the library lays them out: elements per buffer and the row pitch
+        # of each used axis's face grid (axis 2 face rows are pitched)
+        self.face_numel, self.face_pitch = [], []
         for a in host.used_axes if host.absorbing == nat.NW_PML else []:
-            n = S
-            for b in range(3):
-                n *= host.dims[b] + (b == a)
-            self.face_numel.append(-(-n // 64) * 64)  # the library's 64-element rounding
+            elems, pitch = ctypes.c_int64(), ctypes.c_int64()
+            nat.check(lib.nw_face_layout(handle, a, ctypes.byref(elems), ctypes.byref(pitch)),
+                      "nw_face_layout")
+            self.face_numel.append(elems.value)
+            self.face_pitch.append(pitch.value)
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order varies
         h = getattr(self, "handle", None)

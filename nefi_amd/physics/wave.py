@@ -542,10 +542,13 @@ class _LeapfrogBase(Operator):
             out = [prev, h.wavefield_view(state[1], h.n_shots)]
             if P.pml:
                 for a, axis in enumerate(h.used_axes):
+                    # one shot's face grid: extent + 1 along `axis`, rows P.face_pitch[a] apart
+                    # (the faces of the last axis are stored at a pitch past their D + 1)
                     fs = list(self.grid.padded_shape)
                     fs[axis - (3 - self.grid.d)] += 1
-                    n_face = h.n_shots * math.prod(fs)  # buffers are rounded up past this
-                    out.append(psi[a][:n_face].view(h.n_shots, *fs))
+                    stored = fs[:-1] + [P.face_pitch[a]]
+                    n_face = h.n_shots * math.prod(stored)  # buffers are rounded up past this
+                    out.append(psi[a][:n_face].view(h.n_shots, *stored)[..., : fs[-1]])
             return tuple(t.to(device=out_device, dtype=out_dtype) for t in out)
 
         obs = []
