@@ -1646,9 +1646,34 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
       const int y = y0 + r0 + k;
       pp[k] = (act && y < D1) ? ld4(io.p_prev + base + (y * P2 + x)) : f4(0.f);
     }
-    __syncthreads();  // the previous shot's reads of lp are done
+    // The face fields the wave's rows read, issued with the tile so that a shot costs one
+    // memory round trip: y-face rows y0+r0 .. y0+r0+my_rows (cell row k reads face rows k and
+    // k + 1) and each cell row's x-faces. A row whose lane lies in the quiet box reads no faces;
+    // faces skipped or read for such rows hold zero there.
+    float4 oyv[Q ? 1 : S::my_rows + 1], oxv[Q ? 1 : S::my_rows];
+    float oxe[Q ? 1 : S::my_rows];
+    if constexpr (!Q) {
+      auto row_faces = [&](int k) {
+        const int y = y0 + r0 + k;
+        return act && y < D1 && !(col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1]);
+      };
+#pragma unroll
+      for (int j = 0; j <= S::my_rows; ++j) {
+        const int y = y0 + r0 + j;
+        const bool used = (j < S::my_rows && row_faces(j)) || (j > 0 && row_faces(j - 1));
+        oyv[j] = used ? ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x)) : f4(0.f);
+      }
+#pragma unroll
+      for (int k = 0; k < S::my_rows; ++k) {
+        const float* ox_row = io.psi_old[2] + fx + (int64_t)(y0 + r0 + k) * F2 + x;
+        const bool faces = row_faces(k);
+        oxv[k] = faces ? ld4(ox_row) : f4(0.f);
+        oxe[k] = (faces && own_edge && x + 4 <= D2) ? ox_row[4] : 0.f;
+      }
+    }
+    lds_barrier();  // the previous shot's reads of lp are done
     tile_store<TY, R>(lp, v);
-    __syncthreads();
+    lds_barrier();
     float4 w[2 * R + 1];
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) w[k] = lrow(lp, r0 + k, lane);
@@ -1667,9 +1692,7 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
           const float ay0 = ldc(g.alpha[1], y), ay1 = ldc(g.alpha[1], y + 1);
           const float ky0 = ldc(g.kappa[1], y), ky1 = ldc(g.kappa[1], y + 1);
           const float zy0 = ldc(g.zf[1], y), zy1 = ldc(g.zf[1], y + 1);
-          const float4 oy0 = lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)y * D2 + x));
-          const float4 oy1 =
-              lq ? f4(0.f) : ld4(io.psi_old[1] + fy + ((int64_t)(y + 1) * D2 + x));
+          const float4 oy0 = oyv[k], oy1 = oyv[k + 1];
           float4 ny0, ny1;
 #define NW_PML_Y(c)                                                                          \
   {                                                                                          \
@@ -1684,10 +1707,8 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
             if (y == D1 - 1) st4(io.psi_new[1] + fy + ((int64_t)(y + 1) * D2 + x), ny1);
           }
           // x faces x..x+4 of row y
-          const float* ox_row = io.psi_old[2] + fx + (int64_t)y * F2 + x;
           float ox[5], nx[5];
-          faces5(lq ? f4(0.f) : ld4(ox_row), (!lq && own_edge && x + 4 <= D2) ? ox_row[4] : 0.f,
-                 own_edge, ox);
+          faces5(oxv[k], oxe[k], own_edge, ox);
           const float pf[6] = {nb.y, pc.x, pc.y, pc.z, pc.w, nb.z};  // p at x-1 .. x+4
 #pragma unroll
           for (int j = 0; j < 5; ++j)
