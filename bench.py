@@ -20,7 +20,11 @@ stencil + 12 B imaging) times the cell-updates of one launch (a temporally block
 launch advances several steps) over its average duration, timed with HIP events around every
 launch. Temporal blocking moves fewer bytes than that algorithmic count, so for the blocked
 kernel ``achieved`` is an effective rate and ``traffic`` (measured HBM bytes) shows the real
-one.
+one: ``measured_gbs`` / ``measured_frac`` are those bytes over the same launch time. The shot
+groups of the reverse step also share the model tile and the float64 gradient slice, so its
+effective ``frac`` can exceed 1 while the measured one stays below. ``gradient`` gives the whole
+gradient's effective rate at the algorithmic bytes of a fwd + adj cell-update (60 B sponge,
+108 B PML: forward, recompute, adjoint stencil and imaging, no blocking or sharing).
 ``cpu_baseline`` times the reference's CPU path (its op sequence in PyTorch on the CPU, float32,
 all host threads; one thread and the FP64 oracle beside it) on a bounded sample of the same
 workload on this machine's host cores.
@@ -49,6 +53,9 @@ BYTES_PER_UPDATE = {"step": 16.0, "steps_blocked": 16.0, "adjoint": 28.0,
 # with kept Laplacians (the PML default) one kept value instead of p^n and the psi levels (40 B)
 BYTES_PER_UPDATE_PML = {"step": 28.0, "adjoint": 52.0}
 BYTES_PER_UPDATE_PML_KEPT_LAP = {"step": 28.0, "adjoint": 40.0}
+# a whole gradient's algorithmic bytes per fwd + adj cell-update: forward sweep, recompute sweep,
+# adjoint stencil and imaging (16 + 16 + 16 + 12 B; PML 32 + 32 + 32 + 12 B)
+GRADIENT_BYTES = {"sponge": 60.0, "cerjan": 60.0, "pml": 108.0}
 
 
 def survey(n=1024, shots=64, n_t=4001, seed=0, absorbing="sponge"):
@@ -840,6 +847,13 @@ def main() -> int:
         "steps_per_launch": round(steps_per_launch, 3),
         "avg_launch_us": round(avg_s * 1e6, 2),
     }
+    if traffic and avg_s == avg_s:
+        roofline["measured_gbs"] = round(traffic / avg_s / 1e9, 1)
+        roofline["measured_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+    gbytes = GRADIENT_BYTES[op.absorbing]
+    gradient_roof = {"bytes_per_cell_update": gbytes,
+                     "effective_gbs": round(value * gbytes, 1),
+                     "effective_frac": round(value * gbytes / HBM_PEAK_GBS, 4)}
     kernel_ms = {k: {"total_ms": round(v[0] / args.steps, 2), "launches": v[1] // args.steps,
                      "steps": units[k] // args.steps}
                  for k, v in timing.items()}
@@ -896,6 +910,7 @@ def main() -> int:
             "cell_updates_per_step": updates,
         },
         "roofline": roofline,
+        "gradient": gradient_roof,
         "kernels_per_gradient": kernel_ms,
         "tiles": tiles,
         "cpu_baseline": cpu,
