@@ -561,6 +561,7 @@ struct nw_plan {
   float* tb_state[2];
   float* tapbuf;
   int2* tap_rows;  // per shot: first and last grid row of its receiver taps (blocked sweeps)
+  int2* inj_rows;  // per shot: first and last grid row of its injection cells (blocked reverse)
   int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
   int fast3d;        // NW_OPT_FAST_3D: the 2.5-D tiled 3-D float32 kernels (when they cover it)
   int fast64;        // NW_OPT_FAST_F64: the LDS-tiled 2-D float64 forward step
@@ -1130,6 +1131,7 @@ static int adjoint_blocked(nw_plan* P, const float* ma, void** lam, int n_begin,
         io.inj_first = d.inj_first;
         io.inj_count = d.inj_count;
         io.inj_cell = d.inj_cell;
+        io.inj_rows = reinterpret_cast<const int32_t*>(P->inj_rows);
       }
     }
     const int pid = prof_begin(NW_K_ADJOINT_STEPS, st, L);
@@ -1422,6 +1424,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->tb_state[0] = p->tb_state[1] = nullptr;
   p->tapbuf = nullptr;
   p->tap_rows = nullptr;
+  p->inj_rows = nullptr;
   p->keep_lap_opt = -1;
   p->fast3d = fast3d_supported(p->gf, d.absorbing, d.dtype) ? 1 : 0;
   p->fast64 = fast2d64_supported(p->gd, d.absorbing, d.dtype) ? 1 : 0;
@@ -1460,6 +1463,7 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
     if (b) (void)hipFree(b);
   if (plan->tap_rows) (void)hipFree(plan->tap_rows);
+  if (plan->inj_rows) (void)hipFree(plan->inj_rows);
   if (plan->wq_items) (void)hipFree(plan->wq_items);
   if (plan->wq.ctr) (void)hipFree(plan->wq.ctr);
   for (size_t i = 0; i < nw::g_plans.size(); ++i)
@@ -1476,33 +1480,46 @@ int64_t nw_grad_slices(const nw_plan* P) {
   return P->d.n_shots;
 }
 
-// First and last grid row of every shot's receiver taps (an empty range for a shot without
-// receivers): a blocked forward tile scans a shot's taps only when they reach its output rows.
-// Reads the receiver tables back once, when the blocking buffers are allocated.
-static int build_tap_rows(nw_plan* P) {
-  const nw_desc& d = P->d;
-  const int S = d.n_shots;
+// First and last grid row of the cells each shot's table entries [first[s], first[s] +
+// count[s]) point at (`per` cells per entry; an empty range for a shot without entries), as a
+// device array of one int2 per shot.
+static int shot_rows(const nw_plan* P, const int32_t* first_d, const int32_t* count_d,
+                     const int32_t* cells_d, int per, int2** out) {
+  const int S = P->d.n_shots;
   std::vector<int2> tr(S, make_int2(INT32_MAX, -1));
-  if (d.rec_max > 0) {
+  if (first_d && count_d && cells_d) {
     std::vector<int32_t> first(S), count(S);
-    NW_HIP(hipMemcpy(first.data(), d.rec_first, S * sizeof(int32_t), hipMemcpyDefault));
-    NW_HIP(hipMemcpy(count.data(), d.rec_count, S * sizeof(int32_t), hipMemcpyDefault));
+    NW_HIP(hipMemcpy(first.data(), first_d, S * sizeof(int32_t), hipMemcpyDefault));
+    NW_HIP(hipMemcpy(count.data(), count_d, S * sizeof(int32_t), hipMemcpyDefault));
     int64_t rows = 0;
     for (int s = 0; s < S; ++s) rows = std::max<int64_t>(rows, (int64_t)first[s] + count[s]);
-    std::vector<int32_t> idx((size_t)rows * d.rec_taps);
+    std::vector<int32_t> idx((size_t)rows * per);
     if (rows > 0)
-      NW_HIP(hipMemcpy(idx.data(), d.rec_idx, idx.size() * sizeof(int32_t), hipMemcpyDefault));
+      NW_HIP(hipMemcpy(idx.data(), cells_d, idx.size() * sizeof(int32_t), hipMemcpyDefault));
     const int64_t P2 = P->gf.P2;
     for (int s = 0; s < S; ++s)
-      for (int64_t e = (int64_t)first[s] * d.rec_taps; e < ((int64_t)first[s] + count[s]) * d.rec_taps;
-           ++e) {
+      for (int64_t e = (int64_t)first[s] * per; e < ((int64_t)first[s] + count[s]) * per; ++e) {
         const int y = (int)(idx[e] / P2);
         tr[s].x = std::min(tr[s].x, y);
         tr[s].y = std::max(tr[s].y, y);
       }
   }
-  NW_HIP(hipMalloc((void**)&P->tap_rows, S * sizeof(int2)));
-  NW_HIP(hipMemcpy(P->tap_rows, tr.data(), S * sizeof(int2), hipMemcpyHostToDevice));
+  NW_HIP(hipMalloc((void**)out, S * sizeof(int2)));
+  NW_HIP(hipMemcpy(*out, tr.data(), S * sizeof(int2), hipMemcpyHostToDevice));
+  return NW_OK;
+}
+
+// Row ranges of every shot's receiver taps and injection cells: a blocked forward tile scans a
+// shot's taps only when they reach its output rows, a blocked reverse tile its injection cells
+// only when they reach its region. Reads the tables back once, when the blocking buffers are
+// allocated.
+static int build_tap_rows(nw_plan* P) {
+  const nw_desc& d = P->d;
+  const bool rec = d.rec_max > 0, inj = d.inj_max > 0;
+  NW_TRY(shot_rows(P, rec ? d.rec_first : nullptr, rec ? d.rec_count : nullptr,
+                   rec ? d.rec_idx : nullptr, d.rec_taps, &P->tap_rows));
+  NW_TRY(shot_rows(P, inj ? d.inj_first : nullptr, inj ? d.inj_count : nullptr,
+                   inj ? d.inj_cell : nullptr, 1, &P->inj_rows));
   return NW_OK;
 }
 

@@ -1152,7 +1152,7 @@ __device__ __forceinline__ void zero_margin_rows(float* lpa, float* lpb) {
     lpb[i] = 0.f;
     lpb[(S::lrows - R) * kLW + i] = 0.f;
   }
-  for (int r = threadIdx.x; r < S::lrows; r += S::threads) {
+  for (int r = R + threadIdx.x; r < S::lrows - R; r += S::threads) {
     st4(lpa + r * kLW, f4(0.f));
     st4(lpa + r * kLW + kLW - 4, f4(0.f));
     st4(lpb + r * kLW, f4(0.f));
@@ -1312,48 +1312,69 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
       dst[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
     }
   };
+  // kept Laplacians of every level for the wavefront's output rows, loaded with the shot's
+  // region so the level loop does not wait on memory (zero where no imaging applies)
+  auto load_laps = [&](int s, float4 (*dst)[S::mr]) {
+    const int64_t base = (int64_t)s * g.cells;
+#pragma unroll
+    for (int lv = 0; lv < T; ++lv) {
+      const float* lapl = io.lap[lv + 1];
+#pragma unroll
+      for (int k = 0; k < S::mr; ++k) {
+        const int c = c0 + k, y = ry0 + c;
+        const bool ok = lv < io.levels && lapl && outx && c >= S::hy && c < S::hy + TY &&
+                        (Q || y < D1);
+        dst[lv][k] = ok ? ld4(lapl + base + (y * P2 + x)) : f4(0.f);
+      }
+    }
+  };
   const int s_first = tl.grp * spw;
   const int s_end = min(shots, s_first + spw);
   float4 pvn[Q ? S::mr : 1];
+  float4 lapn[Q ? T : 1][S::mr];
   if constexpr (Q) {
-    region_async<S, R>(io.lam1 + (int64_t)s_first * g.cells, ry0, rx0, P2, lpa);
+    rows_async<S, R>(io.lam1 + (int64_t)s_first * g.cells, ry0 + c0, x, P2, c0, lpa);
     load_lam2(s_first, pvn);
+    load_laps(s_first, lapn);
   }
   auto shot = [&](auto bsel, int s) {
     constexpr int B = decltype(bsel)::value;
     const int64_t base = (int64_t)s * g.cells;
     float* lp = (Q && B) ? lpb : lpa;
     float4 pv[S::mr], cur[S::mr];  // lambda^{n+2}, lambda^{n+1} of the current level
+    float4 lapv[T][S::mr];         // kept Laplacians of the levels' output rows
     if constexpr (Q) {
-      __builtin_amdgcn_s_waitcnt(0);  // this shot's region and lambda^{nh+1} rows have landed
+      __builtin_amdgcn_s_waitcnt(0);  // this shot's region, rows and Laplacians have landed
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < S::mr; ++k) pv[k] = pvn[k];
+#pragma unroll
+      for (int lv = 0; lv < T; ++lv)
+#pragma unroll
+        for (int k = 0; k < S::mr; ++k) lapv[lv][k] = lapn[lv][k];
     } else {
-      float4 v[S::per_thread];
+      // the wavefront's own region rows of lambda^{nh}, each lane its four columns (zero
+      // outside the grid); they are also the lane's current level
       const float* L1 = io.lam1 + base;
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + S::threads * k;
-        const int row = i / (kLW / 4), c4 = i % (kLW / 4);
-        const int gy = ry0 + row, gx = rx0 - 4 + 4 * c4;
-        const bool in = (S::threads * (k + 1) <= S::n4 || i < S::n4);
-        const bool ok = in && gy >= 0 && gy < D1 && gx >= 0 && gx < D2;
-        v[k] = ok ? ld4(L1 + (gy * P2 + gx)) : f4(0.f);
+      for (int k = 0; k < S::mr; ++k) {
+        const int y = ry0 + c0 + k;
+        cur[k] = (inx && y >= 0 && y < D1) ? ld4(L1 + (y * P2 + x)) : f4(0.f);
       }
       load_lam2(s, pv);
+      load_laps(s, lapv);
       __syncthreads();  // the previous shot's reads of lp are done
 #pragma unroll
-      for (int k = 0; k < S::per_thread; ++k) {
-        const int i = threadIdx.x + S::threads * k;
-        if (S::threads * (k + 1) <= S::n4 || i < S::n4)
-          *reinterpret_cast<float4*>(lp + R * kLW + 4 * i) = v[k];
-      }
+      for (int k = 0; k < S::mr; ++k)
+        *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = cur[k];
     }
     // injection cells of this shot inside the region: LDS offset (or -1) and value index
     int ioff[kInjChunks];
     bool any_inj = false;
-    const int n_inj = io.injv ? io.inj_count[s] : 0;
+    int n_inj = io.injv ? io.inj_count[s] : 0;
+    // no scan when the shot's injection cells all lie outside the region's rows (uniform)
+    if (n_inj > 0 && (ldc(io.inj_rows, 2 * s + 1) < ry0 || ldc(io.inj_rows, 2 * s) >= ry0 + S::crow))
+      n_inj = 0;
     const int inj0 = io.injv ? io.inj_first[s] : 0;
 #pragma unroll
     for (int c = 0; c < kInjChunks; ++c) {
@@ -1371,19 +1392,23 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
     if constexpr (!Q) __syncthreads();
     if constexpr (Q) {
       if (s + 1 < s_end) {  // stream the next shot in while this one computes
-        region_async<S, R>(io.lam1 + base + g.cells, ry0, rx0, P2, B ? lpa : lpb);
+        rows_async<S, R>(io.lam1 + base + g.cells, ry0 + c0, x, P2, c0, B ? lpa : lpb);
         load_lam2(s + 1, pvn);
+        load_laps(s + 1, lapn);
       }
-    }
-    // lambda^{nh} of this wavefront's rows, then the stencil operand q = (m a) lambda in place
+      // lambda^{nh} of this wavefront's rows
 #pragma unroll
-    for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+      for (int k = 0; k < S::mr; ++k) cur[k] = lrow(lp, R + c0 + k, lane);
+    }
+    // the stencil operand q = (m a) lambda in place
     const bool inj_tile = __syncthreads_or(any_inj);  // (also: every read of lambda is done)
 #pragma unroll
     for (int k = 0; k < S::mr; ++k)
       *reinterpret_cast<float4*>(lp + (R + c0 + k) * kLW + 4 * lane + 4) = mul4(mm[k], cur[k]);
     lds_barrier();
-    for (int lvl = 1; lvl <= io.levels; ++lvl) {
+#pragma unroll
+    for (int lvl = 1; lvl <= T; ++lvl) {
+      if (lvl > io.levels) break;  // uniform
       const int nk = io.nh - lvl;
       const int tpn = (nk + 1) > io.first_step;
       const float* lapl = io.lap[lvl];
@@ -1409,7 +1434,7 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
           if (!Q && !inx) o = f4(0.f);
           const int c = c0 + k;
           if (lapl && outx && c >= S::hy && c < S::hy + TY && (Q || y < D1)) {
-            const float4 lapp = ld4(lapl + base + (y * P2 + x));
+            const float4 lapp = lapv[lvl - 1][k];
             ga[k].x = __builtin_fma((double)rb.x, (double)lapp.x, ga[k].x);
             ga[k].y = __builtin_fma((double)rb.y, (double)lapp.y, ga[k].y);
             gb[k].x = __builtin_fma((double)rb.z, (double)lapp.z, gb[k].x);
@@ -1492,12 +1517,7 @@ __global__ void __launch_bounds__(64 * kAdjWaves) k5_adjoint(Geo<float> g, AdjBl
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
   const Tile tl = tile_of(div_up(D2, S::out_w), div_up(D1, TY), div_up(shots, spw), xcd_map);
   if (!tl.valid) return;  // uniform over the workgroup
-  for (int i = threadIdx.x; i < R * kLW; i += S::threads) {
-    lpa[i] = 0.f;
-    lpa[(S::lrows - R) * kLW + i] = 0.f;
-    lpb[i] = 0.f;
-    lpb[(S::lrows - R) * kLW + i] = 0.f;
-  }
+  zero_margin_rows<S, R>(lpa, lpb);
   const int ry0 = tl.ry * TY - S::hy, rx0 = tl.cx * S::out_w - S::hx;
   const bool quiet = ry0 >= 0 && ry0 + S::crow <= D1 && rx0 >= 4 && rx0 + 260 <= D2 &&
                      ry0 >= g.quiet[1][0] && ry0 + S::crow <= g.quiet[1][1] &&

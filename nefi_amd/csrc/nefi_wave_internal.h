@@ -159,6 +159,7 @@ struct AdjBlockIO {
   const int32_t* inj_first;
   const int32_t* inj_count;
   const int32_t* inj_cell;
+  const int32_t* inj_rows;          // shot s: first (2s) and last (2s + 1) grid row of its cells
 };
 
 // fast 2-D float32 kernels (nefi_wave_2d.hip)

@@ -405,30 +405,46 @@ def test_sensor_bandwidth_attributes_and_resolution_copies():
 
 
 def _blocked_lds_phases(waves, region_rows, R, levels, shots, quiet):
-    """The LDS accesses of one k4_steps workgroup (nefi_wave_2d.hip, k4_tile) as barrier
-    intervals: per interval, per wave, the (buffer, row) sets read and written, plus the rows
-    an asynchronous region fill (quiet tiles) may write until the next shot's full wait.
-    Row numbers are LDS rows (R zero rows, then the region's computed rows)."""
+    """The LDS accesses of one k4_steps / k4_queue workgroup (nefi_wave_2d.hip, k4_tile) as
+    barrier intervals: per interval, per wave, the (buffer, row, float4 column) sets read and
+    written, plus the elements an asynchronous region fill (quiet tiles) may write until the next
+    shot's full wait. Row numbers are LDS rows (R zero rows, then the region's computed rows,
+    then R zero rows); float4 columns 0 and 65 are the margins, 1..64 lanes 0..63's cells."""
     mr = -(-region_rows // waves)
     crow = waves * mr
+    lrows = crow + 2 * R
     lds_w4 = 66  # float4 per LDS row (kLW / 4)
     threads = 64 * waves
-    n4 = crow * lds_w4
 
     def row(b, r, cols=range(lds_w4)):  # the float4 elements of LDS row r of buffer b
         return {(b, r, c) for c in cols}
 
-    def tile_elems(b, wave):  # elements a wave's lanes store when the region is stored by threads
+    def tile_elems(b, wave):  # a wave's own region rows, lane l at float4 l + 1 (no margins)
         out = set()
-        for lane in range(64):
-            i = wave * 64 + lane
-            while i < n4:
-                out.add((b, R + i // lds_w4, i % lds_w4))
-                i += threads
+        for k in range(mr):
+            out |= row(b, R + wave * mr + k, range(1, 65))
         return out
 
-    phases = []  # list of (reads[w], writes[w], async_rows or None)
+    def zeroing(wave):  # zero_margin_rows: the R rows on either side, then thread r's margins
+        out = set()
+        for b in (0, 1):
+            for i in range(64 * wave, 64 * (wave + 1)):
+                j = i
+                while j < R * lds_w4 * 4:  # float by float, threads strided
+                    out |= {(b, j // (lds_w4 * 4), (j // 4) % lds_w4),
+                            (b, lrows - R + j // (lds_w4 * 4), (j // 4) % lds_w4)}
+                    j += threads
+                r = R + i
+                while r < lrows - R:
+                    out |= {(b, r, 0), (b, r, lds_w4 - 1)}
+                    r += threads
+        return out
+
     buf = lambda s: (s % 2) if quiet else 0  # noqa: E731
+    # the zeroing, and for quiet tiles the first shot's fill issued behind it, before the first
+    # barrier
+    phases = [([set() for _ in range(waves)], [zeroing(w) for w in range(waves)],
+               set().union(*(tile_elems(0, w) for w in range(waves))) if quiet else None)]
     for s in range(shots):
         b = buf(s)
         if not quiet:  # synchronous store of the region after a barrier
