@@ -1274,14 +1274,19 @@ __global__ void __launch_bounds__(BlockWaves<TY>::threads)
 //   workgroup's threads add the precomputed sums at the injection cells inside the region
 //   (halo included, so the redundant cells stay exact), and the owners read it back.
 // The lambda values are bit-identical to single reverse steps followed by k_inject.
-constexpr int kAdjWaves = 8;  // wavefronts of a blocked reverse workgroup (256 registers each)
+// wavefronts of a blocked reverse workgroup: 12 for 16-row tiles (two region rows each at depth
+// 2, within the registers of three wavefronts per SIMD), 8 for 8-row tiles
+template <int TY>
+struct AdjBlockWaves {
+  static constexpr int waves = TY >= 16 ? 12 : 8;
+};
 
 template <int R, int ABS0, int TY, int T, bool Q>
 __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& io, int shots,
                                         int spw, const Tile& tl, float* lpa, float* lpb) {
   constexpr int ABS = Q ? NW_NONE : ABS0;
-  using S = BlockShape<TY, R, T, kAdjWaves>;
-  constexpr int kInjChunks = kTapChunks * 256 / S::threads;
+  using S = BlockShape<TY, R, T, AdjBlockWaves<TY>::waves>;
+  constexpr int kInjChunks = (kTapChunks * 256 + S::threads - 1) / S::threads;
   const int D1 = (int)g.D[1], D2 = (int)g.D[2], P2 = (int)g.P2;
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int ry0 = tl.ry * TY - S::hy;
@@ -1420,7 +1425,10 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
         const int y = ry0 + c0 + k;
         w[2 * R] = lrow(lp, c0 + k + 2 * R, lane);
         float4 o = f4(0.f);
-        if (Q || (y >= 0 && y < D1)) {
+        // rows that can still reach the output tile: [R lvl, rows - R lvl) (uniform); the rim
+        // outside takes zeros, which no later level reads into an output row
+        const bool live = c0 + k >= R * lvl && c0 + k < S::rows - R * lvl;
+        if (live && (Q || (y >= 0 && y < D1))) {
           const float4 nb = lnb(lp, c0 + k + R, lane);
           const float4 lapq = stencil4<R>(g, w, nb.x, nb.y, nb.z, nb.w);
           const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
@@ -1509,9 +1517,9 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
 }
 
 template <int R, int ABS, int TY, int T>
-__global__ void __launch_bounds__(64 * kAdjWaves) k5_adjoint(Geo<float> g, AdjBlockIO io,
+__global__ void __launch_bounds__(64 * AdjBlockWaves<TY>::waves) k5_adjoint(Geo<float> g, AdjBlockIO io,
                                                              int shots, int spw, int xcd_map) {
-  using S = BlockShape<TY, R, T, kAdjWaves>;
+  using S = BlockShape<TY, R, T, AdjBlockWaves<TY>::waves>;
   __shared__ __align__(16) float lpa[S::lrows * kLW];
   __shared__ __align__(16) float lpb[S::lrows * kLW];
   const int D1 = (int)g.D[1], D2 = (int)g.D[2];
@@ -2633,7 +2641,7 @@ int fast2d_steps_queue_items(const Geo<float>& g, int order, int absorbing, int 
 template <int R, int ABS, int TY, int T>
 static void launch_adj_steps_t(const Geo<float>& g, const AdjBlockIO& io, int shots, int spw,
                                int xm, hipStream_t st) {
-  using S = BlockShape<TY, R, T, kAdjWaves>;
+  using S = BlockShape<TY, R, T, AdjBlockWaves<TY>::waves>;
   const int tiles =
       div_up((int)g.D[2], S::out_w) * div_up((int)g.D[1], TY) * div_up(shots, spw);
   hipLaunchKernelGGL((k5_adjoint<R, ABS, TY, T>),
