@@ -212,8 +212,9 @@ int32_t nw_plan_get_tiles(const nw_plan* plan, int32_t kind, int32_t* rows,
  *   steps and the imaging sum inside the launch (lambda bit-identical to single steps; the
  *   float64 imaging sums are added in another order). Used when the reverse tiles have rows of
  *   8 or 16, the imaging term (if any) reads kept Laplacians (NW_OPT_KEEP_LAP) and a shot has
- *   at most 2048 injection cells; other calls step one by one. Shares the plan's blocking
- *   wavefields with NW_OPT_TIME_BLOCK.
+ *   at most 2048 injection cells; other calls step one by one. Setting T > 1 allocates two
+ *   wavefields of its own (not those of NW_OPT_TIME_BLOCK), so a blocked reverse sweep may run
+ *   beside a blocked forward sweep of the same plan on another stream.
  * NW_OPT_FAST_3D: 1 (default where they cover the plan: 3-D, float32, sponge/Cerjan/none,
  *   padded width and row pitch multiples of 4) runs the 2.5-D tiled 3-D kernels for forward
  *   steps and for reverse steps without an imaging term; 0 the generic kernels. Plans they do

@@ -565,9 +565,11 @@ struct nw_plan {
   int keep_lap_opt;  // NW_OPT_KEEP_LAP (-1: the default of the absorbing layer)
   int fast3d;        // NW_OPT_FAST_3D: the 2.5-D tiled 3-D float32 kernels (when they cover it)
   int fast64;        // NW_OPT_FAST_F64: the LDS-tiled 2-D float64 forward step
-  // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth and the
-  // residual-injection sums of a block's recorded levels (shares tb_state with the forward)
+  // temporal blocking of the fast reverse step (NW_OPT_TIME_BLOCK_ADJOINT): depth, its own two
+  // extra wavefields (a reverse sweep may run beside a blocked recompute, which alternates
+  // with tb_state), and the residual-injection sums of a block's recorded levels
   int tblock_adj;
+  float* tb_adj[2];
   float* injv;
   // work queue of the blocked forward launches (NW_OPT_WORK_QUEUE): items for the current
   // forward tiles and depth (built when they are set), counters, and the device's CU count
@@ -1063,13 +1065,13 @@ static int forward_impl(nw_plan* P, const T* m, void** state, void** psi, int n_
 static bool use_blocked_adj(const nw_plan* P, const void* keep, const void* residual) {
   const nw_desc& d = P->d;
   return d.dtype == NW_F32 && d.absorbing != NW_PML && P->tblock_adj > 1 && P->rows[1] > 0 &&
-         P->tb_state[0] &&
+         P->tb_adj[0] &&
          P->injv && fast2d_adj_block_supported(P->rows[1], P->tblock_adj) &&
          (!keep || keep_lap(P)) && (!residual || d.inj_max <= 256 * kTapChunks);
 }
 
 // Reverse sweep in blocks of up to tblock_adj steps (2-D float32 fast path). Each block's two
-// outputs go to the two of {lam pair, tb_state} the block does not read.
+// outputs go to the two of {lam pair, tb_adj} the block does not read.
 static int adjoint_blocked(nw_plan* P, const float* ma, void** lam, int n_begin, int n_end,
                            const float* keep, int keep_begin, const float* residual,
                            int inject_end, double* grad, int first_step, hipStream_t st) {
@@ -1080,7 +1082,7 @@ static int adjoint_blocked(nw_plan* P, const float* ma, void** lam, int n_begin,
   };
   float* a0 = (float*)lam[0];
   float* a1 = (float*)lam[1];
-  float* const bufs[4] = {a0, a1, P->tb_state[0], P->tb_state[1]};
+  float* const bufs[4] = {a0, a1, P->tb_adj[0], P->tb_adj[1]};
   float* L1 = a0;  // lambda^{n}
   float* L2 = a1;  // lambda^{n+1}
   if (residual && inject_end && recorded(d, n_end))
@@ -1422,6 +1424,7 @@ int32_t nw_plan_create(const nw_desc* desc, nw_plan** out) {
   p->lds_tiles[0] = p->lds_tiles[1] = 0;
   p->tblock = 1;
   p->tb_state[0] = p->tb_state[1] = nullptr;
+  p->tb_adj[0] = p->tb_adj[1] = nullptr;
   p->tapbuf = nullptr;
   p->tap_rows = nullptr;
   p->inj_rows = nullptr;
@@ -1460,7 +1463,8 @@ int32_t nw_plan_destroy(nw_plan* plan) {
   if (!plan) return NW_OK;
   drop_graphs(plan);
   if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
-  for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tapbuf, plan->injv})
+  for (float* b : {plan->tb_state[0], plan->tb_state[1], plan->tb_adj[0], plan->tb_adj[1],
+                   plan->tapbuf, plan->injv})
     if (b) (void)hipFree(b);
   if (plan->tap_rows) (void)hipFree(plan->tap_rows);
   if (plan->inj_rows) (void)hipFree(plan->inj_rows);
@@ -1649,6 +1653,9 @@ int32_t nw_plan_set_option(nw_plan* P, int32_t option, int32_t value) {
       NW_HIP(hipMalloc((void**)&P->injv, (size_t)inj * sizeof(float)));
       NW_TRY(build_tap_rows(P));
     }
+    if (value > 1 && option == NW_OPT_TIME_BLOCK_ADJOINT && !P->tb_adj[0])
+      for (int k = 0; k < 2; ++k)
+        NW_HIP(hipMalloc((void**)&P->tb_adj[k], (size_t)P->shot_bytes));
     (option == NW_OPT_TIME_BLOCK ? P->tblock : P->tblock_adj) = value;
     return option == NW_OPT_TIME_BLOCK ? build_queue(P) : NW_OK;
   }

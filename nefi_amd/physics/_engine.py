@@ -544,14 +544,10 @@ class Propagator:
 
     def _overlap_fits(self, ws: Workspace, K: int) -> bool:
         """Overlap needs a second keep buffer in free device memory (``NEFI_AMD_OVERLAP=0``
-        turns it off), and a reverse sweep of one launch per step: temporally blocked reverse
-        launches rotate through the plan's two spare wavefields, which the blocked forward
-        recompute on the other stream uses too."""
+        turns it off). Temporally blocked reverse launches rotate through spare wavefields of
+        their own, so they run beside the blocked forward recompute on the other stream."""
         P = self.plan
         if P.device.type != "cuda" or os.environ.get("NEFI_AMD_OVERLAP", "1") == "0":
-            return False
-        if P.dtype == torch.float32 and P.tiles(1)[0] > 0 and \
-                P.option(nat.OPT_TIME_BLOCK_ADJOINT) > 1:
             return False
         size = (K + 1) * P.keep_slot
         missing = sum(1 for name in ("keep", "keep2")

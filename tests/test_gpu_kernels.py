@@ -482,6 +482,27 @@ def test_overlapped_reverse_sweep_is_bit_identical(absorbing, monkeypatch):
         assert torch.equal(g, grads[0])
 
 
+def test_overlapped_sweep_with_blocked_reverse_is_bit_identical(monkeypatch):
+    """Blocked reverse launches (depth 2, kept Laplacians) rotate through wavefields of their
+    own, so the overlapped sweep (blocked recompute on a second stream) runs beside them and
+    gives exactly the one-stream gradient."""
+    print("[case overlapped sweep, blocked reverse]", flush=True)
+    dom, geo, c, kw = _survey("sponge", 4, n=(70, 300), shots=5, n_t=80)
+    op = ShotBatchWaveOperator(dom, geo, **kw)
+    rows = sum(len(r) for _, r in geo)
+    w = torch.randn(rows, kw["n_t"], device=DEV, generator=torch.Generator(DEV).manual_seed(13))
+    tiles = [(32, 2), (16, 2)]
+    out = []
+    for flag, kept in (("0", "4"), ("1", "2"), ("1", "4")):
+        monkeypatch.setenv("NEFI_AMD_OVERLAP", flag)
+        monkeypatch.setenv("NEFI_AMD_KEEP_BLOCKS", kept)
+        out.append(_grad(op, c, w, tiles=tiles, lds=(1, 1), depth=4, keep_lap=1, depth_adj=2))
+        print(f"overlap {flag} kept blocks {kept}: traces {rel(out[-1][0], out[0][0]):.1e}, "
+              f"gradient {rel(out[-1][1], out[0][1]):.1e}", flush=True)
+    for y, g in out[1:]:
+        assert np.array_equal(y, out[0][0]) and np.array_equal(g, out[0][1])
+
+
 def test_misaligned_buffers_are_refused():
     """The tiled kernels move 16-byte vectors: nw_forward / nw_adjoint refuse a buffer that is
     not 16-byte aligned with an error instead of launching."""
