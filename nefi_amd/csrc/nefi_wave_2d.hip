@@ -857,20 +857,15 @@ struct BlockShape {
   static constexpr int mr = (rows + waves - 1) / waves;  // rows of one wavefront
   static constexpr int crow = waves * mr;               // computed rows
   static constexpr int lrows = crow + 2 * R;  // LDS rows: R zero rows above and below
-  static constexpr int n4 = crow * (kLW / 4);  // float4 loads of one region
-  static constexpr int per_thread = (n4 + threads - 1) / threads;
 };
 
 typedef __attribute__((address_space(1))) void gmem_void;
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Issue the loads of one region of a quiet tile (every element inside the grid) straight into
-// LDS rows R.. of `buf` (global_load_lds: lane l of a wavefront lands 16 bytes after lane l-1,
-// from the wavefront's base address). Completion is awaited with s_waitcnt before use.
-// threadIdx.x as a value the compiler cannot see through: per-thread index arithmetic of the
-// tile loads (row and column of each of a thread's float4 elements) is then recomputed where
-// it is used instead of being hoisted out of the shot and item loops, where its long live
-// ranges pushed the blocked kernels over their register budget (scratch spills).
+// threadIdx.x as a value the compiler cannot see through: per-thread index arithmetic (the
+// lane's column, its receiver-tap slots) is then recomputed where it is used instead of being
+// hoisted out of the shot and item loops, where its long live ranges pushed the blocked
+// kernels over their register budget (scratch spills).
 __device__ __forceinline__ int thread_index() {
   int t = threadIdx.x;
   asm volatile("" : "+v"(t));
@@ -890,24 +885,6 @@ __device__ __forceinline__ void rows_async(const float* P, int y, int x, int P2,
     const float* src = P + ((y + k) * P2 + x);
     float* dst = buf + (R + c0 + k) * kLW + 4;
     __builtin_amdgcn_global_load_lds((gmem_void*)src, (lds_void*)dst, 16, 0, 0);
-  }
-}
-
-// The same for the whole region, margins included, with the workgroup's threads spread over
-// its float4 elements (blocked reverse steps).
-template <typename S, int R>
-__device__ __forceinline__ void region_async(const float* P, int ry0, int rx0, int P2,
-                                             float* buf) {
-  const int wbase = thread_index() & ~63;
-#pragma unroll
-  for (int k = 0; k < S::per_thread; ++k) {
-    const int i = thread_index() + S::threads * k;
-    if (S::threads * (k + 1) <= S::n4 || i < S::n4) {
-      const int row = i / (kLW / 4), c4 = i % (kLW / 4);
-      const float* src = P + ((ry0 + row) * P2 + rx0 - 4 + 4 * c4);
-      float* dst = buf + R * kLW + 4 * (wbase + S::threads * k);
-      __builtin_amdgcn_global_load_lds((gmem_void*)src, (lds_void*)dst, 16, 0, 0);
-    }
   }
 }
 
