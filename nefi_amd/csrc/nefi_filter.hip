@@ -12,8 +12,11 @@
 // sequence: an in-place decimation-in-frequency FFT in LDS leaves the spectrum in bit-reversed
 // order, the product with G (real) keeps the two traces apart, and an in-place decimation-in-time
 // inverse FFT takes the bit-reversed spectrum back to natural order, so no permutation pass is
-// needed. The residual (x - x_sub), the mask, the output scale and the float64 sums of the
-// masked squares are fused into the load and store passes.
+// needed. Radix-2 stages run in groups in registers, up to five per LDS round trip; the last
+// forward stages, the product with G and the first inverse stages share one. The residual
+// (x - x_sub) is formed in the first forward pass, which reads the traces from global memory; the
+// mask, the output scale and the float64 sums of the masked squares are fused into the last
+// inverse pass, which writes the outputs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,7 +49,7 @@ constexpr int kFilterBlocks = 2048;  // workgroups of one apply (a loop over row
 static_assert(2 * kFilterBlocks <= NW_FILTER_NOALLOC_PARTIALS,
               "the allocation-free apply's array must hold every workgroup's partial sums");
 constexpr int kMinLog2 = 6;
-// FFT lengths: 16384 float2 (+1/8 padding: 144 KiB of LDS) or 8192 double2 (144 KiB)
+// FFT lengths: 16384 float2 (+1/32 padding: 132 KiB of LDS) or 8192 double2 (+1/16: 136 KiB)
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
 template <typename T>
@@ -114,49 +117,116 @@ __device__ __forceinline__ void block_sums(double v, double m, double* out) {
   }
 }
 
-// LDS slot of sequence element e: one pad slot after every 8 elements, so the stride-8 group
-// accesses of the passes with small spans fall on different banks
-__device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
+// Stages of the forward transform, in passes: the top pass takes R_TOP stages, each middle pass
+// R_MAX, and the merged pass (the last forward stages, the product with the spectrum and the
+// first inverse stages, all in registers) R_LAST. Float32: R_LAST 5, passes of up to 4; float64
+// (twice the registers per element): 4 and 3. Every pass is one LDS round trip: 4 per trace
+// pair for L = 8192 in float32 (4 + 4 + 5 stages), where one pass per stage would take 13.
+template <typename T, int LOG2L>
+struct FftPlan {
+  static constexpr int R_LAST = sizeof(T) == 4 ? 5 : 4;
+  static constexpr int R_MAX = sizeof(T) == 4 ? 4 : 3;
+  static constexpr int REST = LOG2L - R_LAST;                   // >= 1 (LOG2L >= 6)
+  static constexpr int N_MID = (REST + R_MAX - 1) / R_MAX - 1;  // middle passes
+  static constexpr int R_TOP = REST - R_MAX * N_MID;            // 1 .. R_MAX
+  static_assert(REST >= 1 && R_TOP >= 1 && R_TOP <= R_MAX, "pass plan");
+};
 
-// exp(-pi i m / 2^k) for the offsets m < 2^k <= 4 of a group stage (constants after unrolling)
+// LDS slot of sequence element e: one pad slot after every 2^R_LAST elements, so the merged
+// pass's lanes (each on 2^R_LAST consecutive elements) start on different banks; the other
+// passes' lanes walk consecutive elements.
+template <typename T, int LOG2L>
+__device__ __forceinline__ int slot(int e) {
+  return e + (e >> FftPlan<T, LOG2L>::R_LAST);
+}
+
+// exp(-2 pi i j / 32) for j < 16 (cos, -sin): the constant factors of a group's twiddles
+constexpr double kRoot32[16][2] = {
+    {1.0, 0.0},
+    {0.98078528040323044913, -0.19509032201612826785},
+    {0.92387953251128675613, -0.38268343236508977173},
+    {0.83146961230254523708, -0.55557023301960222474},
+    {0.70710678118654752440, -0.70710678118654752440},
+    {0.55557023301960222474, -0.83146961230254523708},
+    {0.38268343236508977173, -0.92387953251128675613},
+    {0.19509032201612826785, -0.98078528040323044913},
+    {0.0, -1.0},
+    {-0.19509032201612826785, -0.98078528040323044913},
+    {-0.38268343236508977173, -0.92387953251128675613},
+    {-0.55557023301960222474, -0.83146961230254523708},
+    {-0.70710678118654752440, -0.70710678118654752440},
+    {-0.83146961230254523708, -0.55557023301960222474},
+    {-0.92387953251128675613, -0.38268343236508977173},
+    {-0.98078528040323044913, -0.19509032201612826785}};
+
+// exp(-pi i m / 2^k) for the offsets m < 2^k <= 16 of a group stage (k <= 4)
 template <typename V>
 __device__ __forceinline__ V unit_root(int k, int m) {
-  constexpr double c = 0.70710678118654752440;
   V r;
-  r.x = 1;
-  r.y = 0;
-  if (k == 1 && m == 1) {
-    r.x = 0;
-    r.y = -1;
-  } else if (k == 2 && m == 1) {
-    r.x = c;
-    r.y = -c;
-  } else if (k == 2 && m == 2) {
-    r.x = 0;
-    r.y = -1;
-  } else if (k == 2 && m == 3) {
-    r.x = -c;
-    r.y = -c;
-  }
+  const int j = m << (4 - k);
+  r.x = kRoot32[j][0];
+  r.y = kRoot32[j][1];
   return r;
 }
 
-// R consecutive radix-2 stages done in registers. The stages of spans 2^lh_top down to
-// s = 2^(lh_top - R + 1) (forward) or s up to 2^lh_top (inverse) only combine the 2^R elements
-// base + q s, q < 2^R, of a group (base = block of 2^(lh_top + 1) elements + offset < s), so a
-// thread loads its groups, runs the R stages and stores them: one LDS round trip and one
-// barrier per R stages. Forward stages are decimation in frequency ((a + b, (a - b) w)),
-// inverse stages decimation in time ((a + b conj(w), a - b conj(w))), with
-// w = exp(-2 pi i j / (2h)) for the element's offset j in its half-block of span h. In a group
-// j = off + m s (m < h / s), so w is the table's exp(-2 pi i off / (2h)) times the constant
-// exp(-pi i m s / h): one twiddle load per stage and group.
+// The R consecutive radix-2 stages of one group, in registers. A group is the 2^R elements
+// base + q s (q < 2^R) of a block of 2^(ls + R) elements, s = 2^ls: the stages of spans
+// s 2^(R-1) down to s (forward) or s up to s 2^(R-1) (inverse) combine only them. Forward
+// stages are decimation in frequency ((a + b, (a - b) w)), inverse stages decimation in time
+// ((a + b conj(w), a - b conj(w))), with w = exp(-2 pi i j / (2h)) for the element's offset j in
+// its half-block of span h. In a group j = off + m s (m < h / s), off = base mod s, so w is the
+// table's exp(-2 pi i off / (2h)) times the constant exp(-pi i m s / h): one twiddle load per
+// stage and group, none when the group starts its block (OFF false: off = 0).
+template <typename T, int LOG2L, int R, bool INV, bool OFF>
+__device__ __forceinline__ void group_stages(typename Cplx<T>::type* e,
+                                             const typename Cplx<T>::type* tw, int ls,
+                                             int off) {
+  using V = typename Cplx<T>::type;
+  constexpr int Q = 1 << R;
+  static_assert(R <= 5, "the constant roots cover spans up to 16 in a group");
+  V tb[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (OFF) {
+      tb[k] = tw[off << (LOG2L - ls - k - 1)];
+    } else {
+      tb[k].x = T(1);
+      tb[k].y = T(0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int k = INV ? t : R - 1 - t;  // the stage's span is s 2^k
+    const int d = 1 << k;               // element distance in q
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if (q & d) continue;
+      const int m = q & (d - 1);
+      const V a = e[q];
+      V b = e[q + d];
+      V w;
+      const bool one = !OFF && m == 0;  // w = 1
+      if (!one) w = !OFF ? unit_root<V>(k, m) : m == 0 ? tb[k] : cmul(tb[k], unit_root<V>(k, m));
+      if (INV && !one) b = cmul_conj(b, w);
+      V sum, dif;
+      sum.x = a.x + b.x;
+      sum.y = a.y + b.y;
+      dif.x = a.x - b.x;
+      dif.y = a.y - b.y;
+      e[q] = sum;
+      e[q + d] = (INV || one) ? dif : cmul(dif, w);
+    }
+  }
+}
+
+// One pass of R stages whose top span is 2^lh_top, LDS to LDS, CH groups in flight per thread
 template <typename T, int LOG2L, int R, bool INV>
-__device__ __forceinline__ void fft_group_pass(typename Cplx<T>::type* z,
-                                               const typename Cplx<T>::type* tw, int lh_top) {
+__device__ __forceinline__ void pass_lds(typename Cplx<T>::type* z,
+                                         const typename Cplx<T>::type* tw, int lh_top) {
   using V = typename Cplx<T>::type;
   constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q;
   constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;  // groups per thread
-  constexpr int CH0 = sizeof(T) == 4 ? 4 : 2;
+  constexpr int CH0 = sizeof(T) == 4 ? (R >= 4 ? 2 : 4) : (R >= 3 ? 1 : 2);
   constexpr int CH = KG < CH0 ? KG : CH0;  // groups loaded before the first store
   const int ls = lh_top - (R - 1);
   const int s = 1 << ls;
@@ -169,84 +239,159 @@ __device__ __forceinline__ void fft_group_pass(typename Cplx<T>::type* z,
       const int g = threadIdx.x + kFilterThreads * (c0 + c);
       base[c] = -1;
       if (G % kFilterThreads == 0 || g < G) {
-        const int off = g & (s - 1);
-        base[c] = ((g >> ls) << (ls + R)) + off;
+        base[c] = ((g >> ls) << (ls + R)) + (g & (s - 1));
 #pragma unroll
-        for (int q = 0; q < Q; ++q) e[c][q] = z[pad8(base[c] + q * s)];
+        for (int q = 0; q < Q; ++q) e[c][q] = z[slot<T, LOG2L>(base[c] + q * s)];
       }
-    }
-    V tb[CH][R];  // exp(-2 pi i off / (2h)) for the spans h = s 2^k of the group's stages
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      if (base[c] < 0) continue;
-      const int off = base[c] & (s - 1);
-#pragma unroll
-      for (int k = 0; k < R; ++k) tb[c][k] = tw[off << (LOG2L - ls - k - 1)];
     }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       if (base[c] < 0) continue;
+      group_stages<T, LOG2L, R, INV, true>(e[c], tw, ls, base[c] & (s - 1));
 #pragma unroll
-      for (int t = 0; t < R; ++t) {
-        const int k = INV ? t : R - 1 - t;  // the stage's span is s 2^k
-        const int d = 1 << k;               // element distance in q
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          if (q & d) continue;
-          const int m = q & (d - 1);
-          const V w = m == 0 ? tb[c][k] : cmul(tb[c][k], unit_root<V>(k, m));
-          const V a = e[c][q];
-          const V b = INV ? cmul_conj(e[c][q + d], w) : e[c][q + d];
-          V sum, dif;
-          sum.x = a.x + b.x;
-          sum.y = a.y + b.y;
-          dif.x = a.x - b.x;
-          dif.y = a.y - b.y;
-          e[c][q] = sum;
-          e[c][q + d] = INV ? dif : cmul(dif, w);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < Q; ++q) z[pad8(base[c] + q * s)] = e[c][q];
+      for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(base[c] + q * s)] = e[c][q];
     }
   }
 }
 
-// forward transform (natural order in, bit-reversed out): passes of 3 stages from the top
-// span down, then the remaining 1 or 2; the inverse runs the same groups in reverse order
-template <typename T, int LOG2L, bool INV>
-__device__ __forceinline__ void fft_all(typename Cplx<T>::type* z,
-                                        const typename Cplx<T>::type* tw) {
-  constexpr int P = LOG2L / 3, REM = LOG2L % 3;
-  if constexpr (!INV) {
+// groups of the top passes a thread keeps in flight: 16 loads per trace of the pair
+template <int R, int KG>
+struct TopChunk {
+  static constexpr int CH0 = (R >= 4) ? 1 : (16 >> R);
+  static constexpr int CH = KG < CH0 ? KG : CH0;
+};
+
+// The first forward pass: its groups' elements come straight from the traces (the residual
+// x - x_sub; rows r0, r0 + 1 as the real and imaginary parts, zero past the trace length).
+// Group g holds t = g + q 2^ls: the lanes of a wavefront read consecutive samples.
+template <typename T, int LOG2L, int R>
+__device__ __forceinline__ void pass_top_load(typename Cplx<T>::type* z,
+                                              const typename Cplx<T>::type* tw,
+                                              const FilterIO<T>& io, int64_t o0, int64_t o1,
+                                              bool two) {
+  using V = typename Cplx<T>::type;
+  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q, ls = LOG2L - R;
+  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
+  constexpr int CH = TopChunk<R, KG>::CH;
+  const int n = (int)io.n;
 #pragma unroll 1
-    for (int q = 0; q < P; ++q) {
-      fft_group_pass<T, LOG2L, 3, false>(z, tw, LOG2L - 1 - 3 * q);
-      __syncthreads();
+  for (int c0 = 0; c0 < KG; c0 += CH) {
+    V e[CH][Q];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = threadIdx.x + kFilterThreads * (c0 + c);  // < G = 2^ls: the group's offset
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int t = g + (q << ls);
+        T a = T(0), b = T(0);
+        if ((G % kFilterThreads == 0 || g < G) && t < n) {
+          a = io.x[o0 + t];
+          if (io.xs) a = a - io.xs[o0 + t];
+          if (two) {
+            b = io.x[o1 + t];
+            if (io.xs) b = b - io.xs[o1 + t];
+          }
+        }
+        e[c][q].x = a;
+        e[c][q].y = b;
+      }
     }
-    if constexpr (REM > 0) {
-      fft_group_pass<T, LOG2L, (REM > 0 ? REM : 1), false>(z, tw, REM - 1);
-      __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = threadIdx.x + kFilterThreads * (c0 + c);
+      if (G % kFilterThreads != 0 && g >= G) continue;
+      group_stages<T, LOG2L, R, false, true>(e[c], tw, ls, g);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(g + (q << ls))] = e[c][q];
     }
-  } else {
-    if constexpr (REM > 0) {
-      fft_group_pass<T, LOG2L, (REM > 0 ? REM : 1), true>(z, tw, REM - 1);
-      __syncthreads();
-    }
+  }
+}
+
+// The last inverse pass: its groups' elements go straight to the outputs (mask, scale) and into
+// the float64 sums of the masked squares.
+template <typename T, int LOG2L, int R>
+__device__ __forceinline__ void pass_top_store(const typename Cplx<T>::type* z,
+                                               const typename Cplx<T>::type* tw,
+                                               const FilterIO<T>& io, int64_t o0, int64_t o1,
+                                               bool two, double& acc_v, double& acc_m) {
+  using V = typename Cplx<T>::type;
+  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q, ls = LOG2L - R;
+  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
+  constexpr int CH = TopChunk<R, KG>::CH;
+  const int n = (int)io.n;
 #pragma unroll 1
-    for (int q = P - 1; q >= 0; --q) {
-      fft_group_pass<T, LOG2L, 3, true>(z, tw, LOG2L - 1 - 3 * q);
-      __syncthreads();
+  for (int c0 = 0; c0 < KG; c0 += CH) {
+    V e[CH][Q];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = threadIdx.x + kFilterThreads * (c0 + c);
+      if (G % kFilterThreads != 0 && g >= G) continue;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) e[c][q] = z[slot<T, LOG2L>(g + (q << ls))];
     }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = threadIdx.x + kFilterThreads * (c0 + c);
+      if (G % kFilterThreads != 0 && g >= G) continue;
+      group_stages<T, LOG2L, R, true, true>(e[c], tw, ls, g);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int t = g + (q << ls);
+        if (t < n) {
+          const V v = e[c][q];
+          const T m0 = io.mask ? io.mask[o0 + t] : T(1);
+          if (io.y) io.y[o0 + t] = io.scale * (m0 * v.x);
+          acc_v += (double)m0 * ((double)v.x * (double)v.x);
+          acc_m += (double)m0;
+          if (two) {
+            const T m1 = io.mask ? io.mask[o1 + t] : T(1);
+            if (io.y) io.y[o1 + t] = io.scale * (m1 * v.y);
+            acc_v += (double)m1 * ((double)v.y * (double)v.y);
+            acc_m += (double)m1;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The last forward stages (spans 2^(R-1) .. 1: groups of 2^R consecutive elements), the product
+// with the spectrum (real, at the bit-reversed positions the forward stages leave), and the
+// first inverse stages on the same groups, in registers.
+template <typename T, int LOG2L, int R>
+__device__ __forceinline__ void pass_merged(typename Cplx<T>::type* z,
+                                            const typename Cplx<T>::type* tw,
+                                            const T* spec) {
+  using V = typename Cplx<T>::type;
+  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q;
+  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
+#pragma unroll 1
+  for (int c = 0; c < KG; ++c) {
+    const int g = threadIdx.x + kFilterThreads * c;
+    if (G % kFilterThreads != 0 && g >= G) break;
+    const int base = g << R;
+    V e[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) e[q] = z[slot<T, LOG2L>(base + q)];
+    group_stages<T, LOG2L, R, false, false>(e, tw, 0, 0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const T h = spec[base + q];
+      e[q].x = e[q].x * h;
+      e[q].y = e[q].y * h;
+    }
+    group_stages<T, LOG2L, R, true, false>(e, tw, 0, 0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(base + q)] = e[q];
   }
 }
 
 template <typename T, int LOG2L>
 __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
   using V = typename Cplx<T>::type;
+  using F = FftPlan<T, LOG2L>;
   constexpr int L = 1 << LOG2L;
-  __shared__ V z[L + L / 8];  // padded (pad8)
-  const int n = (int)io.n;
+  __shared__ V z[L + (L >> F::R_LAST)];  // padded (slot)
   double acc_v = 0.0, acc_m = 0.0;
   const int64_t pairs = (io.rows + 1) / 2;
   for (int64_t p = blockIdx.x; p < pairs; p += gridDim.x) {
@@ -254,45 +399,21 @@ __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
     const bool two = r0 + 1 < io.rows;
     const int64_t o0 = r0 * io.n, o1 = two ? o0 + io.n : o0;
     __syncthreads();  // the previous pair's reads of z are done
-    for (int t = threadIdx.x; t < L; t += kFilterThreads) {
-      T a = T(0), b = T(0);
-      if (t < n) {
-        a = io.x[o0 + t];
-        if (io.xs) a = a - io.xs[o0 + t];
-        if (two) {
-          b = io.x[o1 + t];
-          if (io.xs) b = b - io.xs[o1 + t];
-        }
-      }
-      V v;
-      v.x = a;
-      v.y = b;
-      z[pad8(t)] = v;
-    }
+    pass_top_load<T, LOG2L, F::R_TOP>(z, io.tw, io, o0, o1, two);
     __syncthreads();
-    fft_all<T, LOG2L, false>(z, io.tw);  // spectrum in bit-reversed order
-    for (int t = threadIdx.x; t < L; t += kFilterThreads) {
-      const T g = io.spec[t];
-      V v = z[pad8(t)];
-      v.x = v.x * g;
-      v.y = v.y * g;
-      z[pad8(t)] = v;
+#pragma unroll
+    for (int q = 0; q < F::N_MID; ++q) {
+      pass_lds<T, LOG2L, F::R_MAX, false>(z, io.tw, LOG2L - 1 - F::R_TOP - F::R_MAX * q);
+      __syncthreads();
     }
+    pass_merged<T, LOG2L, F::R_LAST>(z, io.tw, io.spec);
     __syncthreads();
-    fft_all<T, LOG2L, true>(z, io.tw);  // back to natural order
-    for (int t = threadIdx.x; t < n; t += kFilterThreads) {
-      const V v = z[pad8(t)];
-      const T m0 = io.mask ? io.mask[o0 + t] : T(1);
-      if (io.y) io.y[o0 + t] = io.scale * (m0 * v.x);
-      acc_v += (double)m0 * ((double)v.x * (double)v.x);
-      acc_m += (double)m0;
-      if (two) {
-        const T m1 = io.mask ? io.mask[o1 + t] : T(1);
-        if (io.y) io.y[o1 + t] = io.scale * (m1 * v.y);
-        acc_v += (double)m1 * ((double)v.y * (double)v.y);
-        acc_m += (double)m1;
-      }
+#pragma unroll
+    for (int q = F::N_MID - 1; q >= 0; --q) {
+      pass_lds<T, LOG2L, F::R_MAX, true>(z, io.tw, LOG2L - 1 - F::R_TOP - F::R_MAX * q);
+      __syncthreads();
     }
+    pass_top_store<T, LOG2L, F::R_TOP>(z, io.tw, io, o0, o1, two, acc_v, acc_m);
   }
   if (io.partial) block_sums(acc_v, acc_m, io.partial + 2 * blockIdx.x);
 }
