@@ -93,6 +93,22 @@ __device__ __forceinline__ V cmul_conj(const V& a, const V& b) {  // a * conj(b)
   return r;
 }
 
+template <typename V>
+__device__ __forceinline__ V mul_neg_i(const V& a) {  // a * (-i)
+  V r;
+  r.x = a.y;
+  r.y = -a.x;
+  return r;
+}
+
+template <typename V>
+__device__ __forceinline__ V mul_i(const V& a) {  // a * i
+  V r;
+  r.x = -a.y;
+  r.y = a.x;
+  return r;
+}
+
 // the two float64 sums of a workgroup (fixed order: lanes by shuffle tree, then wavefronts)
 __device__ __forceinline__ void block_sums(double v, double m, double* out) {
   __shared__ double red[2][kFilterThreads / 64];
@@ -119,17 +135,27 @@ __device__ __forceinline__ void block_sums(double v, double m, double* out) {
 
 // Stages of the forward transform, in passes: the top pass takes R_TOP stages, each middle pass
 // R_MAX, and the merged pass (the last forward stages, the product with the spectrum and the
-// first inverse stages, all in registers) R_LAST. Float32: R_LAST 5, passes of up to 4; float64
-// (twice the registers per element): 4 and 3. Every pass is one LDS round trip: 4 per trace
-// pair for L = 8192 in float32 (4 + 4 + 5 stages), where one pass per stage would take 13.
+// first inverse stages, all in registers) R_LAST. Float32: R_LAST 5, the top pass up to 5 (half
+// of its group is zero padding) and middle passes of 4; float64 (twice the registers per
+// element): 4, 4 and 3. Every pass is one LDS round trip: 3 for L = 8192 in float32
+// (4 + 4 + 5 stages) where one pass per stage would take 13.
 template <typename T, int LOG2L>
 struct FftPlan {
   static constexpr int R_LAST = sizeof(T) == 4 ? 5 : 4;
+  static constexpr int R_TOP_MAX = R_LAST;
   static constexpr int R_MAX = sizeof(T) == 4 ? 4 : 3;
-  static constexpr int REST = LOG2L - R_LAST;                   // >= 1 (LOG2L >= 6)
-  static constexpr int N_MID = (REST + R_MAX - 1) / R_MAX - 1;  // middle passes
-  static constexpr int R_TOP = REST - R_MAX * N_MID;            // 1 .. R_MAX
-  static_assert(REST >= 1 && R_TOP >= 1 && R_TOP <= R_MAX, "pass plan");
+  static constexpr int REST = LOG2L - R_LAST;  // >= 1 (LOG2L >= 6)
+  static constexpr int N_MID = REST <= R_TOP_MAX ? 0 : (REST - R_TOP_MAX + R_MAX - 1) / R_MAX;
+  static constexpr int R_TOP = REST - R_MAX * N_MID;
+  static_assert(REST >= 1 && R_TOP >= 1 && R_TOP <= R_TOP_MAX, "pass plan");
+  // workgroups of 4 wavefronts the LDS admits per CU (160 KiB), i.e. wavefronts per SIMD: the
+  // register cap asks for no more than 2 (256 VGPRs)
+  static constexpr int LDS_BYTES = ((1 << LOG2L) + ((1 << LOG2L) >> R_LAST)) * 2 * (int)sizeof(T);
+  static constexpr int WAVES = (160 * 1024 / LDS_BYTES) >= 2 ? 2 : 1;
+  // the top pass's weights held from the start of a pair (32 VGPRs at most)
+  static constexpr int MASK_REGS = 2 * (((1 << LOG2L) >> R_TOP) + kFilterThreads - 1) /
+                                   kFilterThreads * (1 << (R_TOP - 1)) * (int)sizeof(T) / 4;
+  static constexpr bool MASK_AHEAD = MASK_REGS <= 32;
 };
 
 // LDS slot of sequence element e: one pad slot after every 2^R_LAST elements, so the merged
@@ -202,19 +228,27 @@ __device__ __forceinline__ void group_stages(typename Cplx<T>::type* e,
     for (int q = 0; q < Q; ++q) {
       if (q & d) continue;
       const int m = q & (d - 1);
+      // the constant factor exp(-pi i m / 2^k) is 1 for m = 0 and -i for m = 2^(k-1): those
+      // products are exact swaps and sign flips (bit-identical to the general product)
+      const bool unit = m == 0;
+      const bool neg_i = m != 0 && (m << (4 - k)) == 8;
       const V a = e[q];
       V b = e[q + d];
       V w;
-      const bool one = !OFF && m == 0;  // w = 1
-      if (!one) w = !OFF ? unit_root<V>(k, m) : m == 0 ? tb[k] : cmul(tb[k], unit_root<V>(k, m));
-      if (INV && !one) b = cmul_conj(b, w);
+      if (OFF) {
+        w = unit ? tb[k] : neg_i ? mul_neg_i(tb[k]) : cmul(tb[k], unit_root<V>(k, m));
+      } else if (!unit && !neg_i) {
+        w = unit_root<V>(k, m);
+      }
+      const bool trivial = !OFF && (unit || neg_i);
+      if (INV) b = !trivial ? cmul_conj(b, w) : neg_i ? mul_i(b) : b;
       V sum, dif;
       sum.x = a.x + b.x;
       sum.y = a.y + b.y;
       dif.x = a.x - b.x;
       dif.y = a.y - b.y;
       e[q] = sum;
-      e[q + d] = (INV || one) ? dif : cmul(dif, w);
+      e[q + d] = INV ? dif : !trivial ? cmul(dif, w) : neg_i ? mul_neg_i(dif) : dif;
     }
   }
 }
@@ -254,58 +288,126 @@ __device__ __forceinline__ void pass_lds(typename Cplx<T>::type* z,
   }
 }
 
-// groups of the top passes a thread keeps in flight: 16 loads per trace of the pair
-template <int R, int KG>
-struct TopChunk {
-  static constexpr int CH0 = (R >= 4) ? 1 : (16 >> R);
-  static constexpr int CH = KG < CH0 ? KG : CH0;
+// The top passes' groups: group g < G = 2^ls holds t = g + q 2^ls (q < 2^R), so the lanes of a
+// wavefront touch consecutive samples. Traces are at most L/2 long (L >= 2n - 1), so only the
+// lower half q < 2^(R-1) of a group meets the traces: the upper half is zero on the way in and
+// discarded on the way out.
+template <typename T, int LOG2L, int R>
+struct TopPass {
+  static constexpr int Q = 1 << R, QH = Q / 2, G = (1 << LOG2L) / Q, ls = LOG2L - R;
+  static constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;  // groups per thread
+  __device__ static bool valid(int c) {
+    return G % kFilterThreads == 0 || (int)threadIdx.x + kFilterThreads * c < G;
+  }
 };
 
+// Raw buffer loads over one trace row: a resource of `bytes` bytes (stride 0) returns zero for a
+// load past its end, which is the transform's zero padding, and a resource of 0 bytes stands for
+// the missing second row of an odd row count. Word 3 selects 32-bit data (a zero data format
+// would make every load read zero).
+constexpr int kRsrcWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, bytes, kRsrcWord3);
+}
+
+template <typename T>
+__device__ __forceinline__ T row_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // The first forward pass: its groups' elements come straight from the traces (the residual
-// x - x_sub; rows r0, r0 + 1 as the real and imaginary parts, zero past the trace length).
-// Group g holds t = g + q 2^ls: the lanes of a wavefront read consecutive samples.
+// x - x_sub; rows r0, r0 + 1 as the real and imaginary parts, zero past the trace length). Lane
+// offsets in VGPRs, q offsets in SGPRs, no branches: a group's loads are in flight together.
+template <typename T, int LOG2L, int R, bool SUB>
+__device__ __forceinline__ void pass_top_load_t(typename Cplx<T>::type* z,
+                                                const typename Cplx<T>::type* tw,
+                                                const FilterIO<T>& io, int64_t o0, int64_t o1,
+                                                bool two) {
+  using V = typename Cplx<T>::type;
+  using P = TopPass<T, LOG2L, R>;
+  const int bytes = (int)io.n * (int)sizeof(T);
+  const auto x0 = row_rsrc(io.x + o0, bytes);
+  const auto x1 = row_rsrc(io.x + o1, two ? bytes : 0);
+  const auto s0 = row_rsrc(SUB ? io.xs + o0 : io.x, SUB ? bytes : 0);
+  const auto s1 = row_rsrc(SUB ? io.xs + o1 : io.x, SUB && two ? bytes : 0);
+#pragma unroll
+  for (int c = 0; c < P::KG; ++c) {
+    if (!P::valid(c)) continue;
+    const int g = threadIdx.x + kFilterThreads * c;
+    const int voff = g * (int)sizeof(T);
+    T a[P::QH], b[P::QH], sa[P::QH], sb[P::QH];
+#pragma unroll
+    for (int q = 0; q < P::QH; ++q) {
+      const int soff = (q << P::ls) * (int)sizeof(T);
+      a[q] = row_load<T>(x0, voff, soff);
+      b[q] = row_load<T>(x1, voff, soff);
+      if (SUB) {
+        sa[q] = row_load<T>(s0, voff, soff);
+        sb[q] = row_load<T>(s1, voff, soff);
+      }
+    }
+    // every load of the group issued before the first use (the scheduler would otherwise
+    // interleave them with the arithmetic, one memory latency per sample)
+    __builtin_amdgcn_sched_barrier(0);
+    V e[P::Q];
+#pragma unroll
+    for (int q = 0; q < P::QH; ++q) {
+      e[q].x = SUB ? a[q] - sa[q] : a[q];
+      e[q].y = SUB ? b[q] - sb[q] : b[q];
+      e[q + P::QH].x = T(0);
+      e[q + P::QH].y = T(0);
+    }
+    group_stages<T, LOG2L, R, false, true>(e, tw, P::ls, g);
+#pragma unroll
+    for (int q = 0; q < P::Q; ++q) z[slot<T, LOG2L>(g + (q << P::ls))] = e[q];
+  }
+}
+
 template <typename T, int LOG2L, int R>
 __device__ __forceinline__ void pass_top_load(typename Cplx<T>::type* z,
                                               const typename Cplx<T>::type* tw,
                                               const FilterIO<T>& io, int64_t o0, int64_t o1,
                                               bool two) {
-  using V = typename Cplx<T>::type;
-  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q, ls = LOG2L - R;
-  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
-  constexpr int CH = TopChunk<R, KG>::CH;
-  const int n = (int)io.n;
-#pragma unroll 1
-  for (int c0 = 0; c0 < KG; c0 += CH) {
-    V e[CH][Q];
+  if (io.xs)
+    pass_top_load_t<T, LOG2L, R, true>(z, tw, io, o0, o1, two);
+  else
+    pass_top_load_t<T, LOG2L, R, false>(z, tw, io, o0, o1, two);
+}
+
+// The weights of the samples the last inverse pass writes, loaded ahead (at the start of the
+// pair) so their latency hides behind the LDS passes: the mask, 1 without one, and 0 past the
+// trace length and for the missing second row, so the sums need no tests.
+template <typename T, int LOG2L, int R>
+struct TopMask {
+  using P = TopPass<T, LOG2L, R>;
+  T m[2][P::KG][P::QH];
+  __device__ __forceinline__ void load(const FilterIO<T>& io, int64_t o0, int64_t o1, bool two) {
+    const int n = (int)io.n, bytes = n * (int)sizeof(T);
+    const auto m0 = row_rsrc(io.mask ? io.mask + o0 : io.x, io.mask ? bytes : 0);
+    const auto m1 = row_rsrc(io.mask ? io.mask + o1 : io.x, io.mask && two ? bytes : 0);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int g = threadIdx.x + kFilterThreads * (c0 + c);  // < G = 2^ls: the group's offset
+    for (int c = 0; c < P::KG; ++c) {
+      const int g = threadIdx.x + kFilterThreads * c;
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int t = g + (q << ls);
-        T a = T(0), b = T(0);
-        if ((G % kFilterThreads == 0 || g < G) && t < n) {
-          a = io.x[o0 + t];
-          if (io.xs) a = a - io.xs[o0 + t];
-          if (two) {
-            b = io.x[o1 + t];
-            if (io.xs) b = b - io.xs[o1 + t];
-          }
+      for (int q = 0; q < P::QH; ++q) {
+        const int t = g + (q << P::ls);
+        if (io.mask) {
+          const int voff = g * (int)sizeof(T), soff = (q << P::ls) * (int)sizeof(T);
+          m[0][c][q] = row_load<T>(m0, voff, soff);
+          m[1][c][q] = row_load<T>(m1, voff, soff);
+        } else {
+          m[0][c][q] = t < n ? T(1) : T(0);
+          m[1][c][q] = t < n && two ? T(1) : T(0);
         }
-        e[c][q].x = a;
-        e[c][q].y = b;
       }
     }
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int g = threadIdx.x + kFilterThreads * (c0 + c);
-      if (G % kFilterThreads != 0 && g >= G) continue;
-      group_stages<T, LOG2L, R, false, true>(e[c], tw, ls, g);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(g + (q << ls))] = e[c][q];
-    }
   }
-}
+};
 
 // The last inverse pass: its groups' elements go straight to the outputs (mask, scale) and into
 // the float64 sums of the masked squares.
@@ -313,43 +415,29 @@ template <typename T, int LOG2L, int R>
 __device__ __forceinline__ void pass_top_store(const typename Cplx<T>::type* z,
                                                const typename Cplx<T>::type* tw,
                                                const FilterIO<T>& io, int64_t o0, int64_t o1,
-                                               bool two, double& acc_v, double& acc_m) {
+                                               bool two, const TopMask<T, LOG2L, R>& mk,
+                                               double& acc_v, double& acc_m) {
   using V = typename Cplx<T>::type;
-  constexpr int L = 1 << LOG2L, Q = 1 << R, G = L / Q, ls = LOG2L - R;
-  constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
-  constexpr int CH = TopChunk<R, KG>::CH;
+  using P = TopPass<T, LOG2L, R>;
   const int n = (int)io.n;
-#pragma unroll 1
-  for (int c0 = 0; c0 < KG; c0 += CH) {
-    V e[CH][Q];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int g = threadIdx.x + kFilterThreads * (c0 + c);
-      if (G % kFilterThreads != 0 && g >= G) continue;
+  for (int c = 0; c < P::KG; ++c) {
+    if (!P::valid(c)) continue;
+    const int g = threadIdx.x + kFilterThreads * c;
+    V e[P::Q];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) e[c][q] = z[slot<T, LOG2L>(g + (q << ls))];
-    }
+    for (int q = 0; q < P::Q; ++q) e[q] = z[slot<T, LOG2L>(g + (q << P::ls))];
+    group_stages<T, LOG2L, R, true, true>(e, tw, P::ls, g);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int g = threadIdx.x + kFilterThreads * (c0 + c);
-      if (G % kFilterThreads != 0 && g >= G) continue;
-      group_stages<T, LOG2L, R, true, true>(e[c], tw, ls, g);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int t = g + (q << ls);
-        if (t < n) {
-          const V v = e[c][q];
-          const T m0 = io.mask ? io.mask[o0 + t] : T(1);
-          if (io.y) io.y[o0 + t] = io.scale * (m0 * v.x);
-          acc_v += (double)m0 * ((double)v.x * (double)v.x);
-          acc_m += (double)m0;
-          if (two) {
-            const T m1 = io.mask ? io.mask[o1 + t] : T(1);
-            if (io.y) io.y[o1 + t] = io.scale * (m1 * v.y);
-            acc_v += (double)m1 * ((double)v.y * (double)v.y);
-            acc_m += (double)m1;
-          }
-        }
+    for (int q = 0; q < P::QH; ++q) {
+      const int t = g + (q << P::ls);
+      const V v = e[q];
+      const T m0 = mk.m[0][c][q], m1 = mk.m[1][c][q];
+      acc_v += (double)m0 * ((double)v.x * (double)v.x) + (double)m1 * ((double)v.y * (double)v.y);
+      acc_m += (double)m0 + (double)m1;
+      if (io.y && t < n) {
+        io.y[o0 + t] = io.scale * (m0 * v.x);
+        if (two) io.y[o1 + t] = io.scale * (m1 * v.y);
       }
     }
   }
@@ -387,7 +475,8 @@ __device__ __forceinline__ void pass_merged(typename Cplx<T>::type* z,
 }
 
 template <typename T, int LOG2L>
-__global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
+__global__ void __launch_bounds__(kFilterThreads)
+    __attribute__((amdgpu_waves_per_eu(FftPlan<T, LOG2L>::WAVES))) k_filter_fft(FilterIO<T> io) {
   using V = typename Cplx<T>::type;
   using F = FftPlan<T, LOG2L>;
   constexpr int L = 1 << LOG2L;
@@ -400,6 +489,8 @@ __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
     const int64_t o0 = r0 * io.n, o1 = two ? o0 + io.n : o0;
     __syncthreads();  // the previous pair's reads of z are done
     pass_top_load<T, LOG2L, F::R_TOP>(z, io.tw, io, o0, o1, two);
+    TopMask<T, LOG2L, F::R_TOP> mk;
+    if (F::MASK_AHEAD) mk.load(io, o0, o1, two);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < F::N_MID; ++q) {
@@ -413,7 +504,8 @@ __global__ void __launch_bounds__(kFilterThreads) k_filter_fft(FilterIO<T> io) {
       pass_lds<T, LOG2L, F::R_MAX, true>(z, io.tw, LOG2L - 1 - F::R_TOP - F::R_MAX * q);
       __syncthreads();
     }
-    pass_top_store<T, LOG2L, F::R_TOP>(z, io.tw, io, o0, o1, two, acc_v, acc_m);
+    if (!F::MASK_AHEAD) mk.load(io, o0, o1, two);
+    pass_top_store<T, LOG2L, F::R_TOP>(z, io.tw, io, o0, o1, two, mk, acc_v, acc_m);
   }
   if (io.partial) block_sums(acc_v, acc_m, io.partial + 2 * blockIdx.x);
 }

@@ -115,6 +115,35 @@ def test_band_limited_mse_value_and_adjoint_source(f_max, relative, mask_kind):
         assert loss.dtype == dtype and el < tol and eg < tol
 
 
+@pytest.mark.parametrize("n", [1, 33, 100, 200, 400, 600, 1500, 3000, 4096, 5000, 8192])
+def test_every_transform_length(n):
+    """Every FFT length of the plans (L = 64 .. 16384 for float32, .. 8192 for float64: each has
+    its own pass plan), through the fused misfit (residual, full mask, sums) with an odd row
+    count (the last row of the pair is absent) and through the plain filter."""
+    print(f"[case transform length n={n}]", flush=True)
+    rng = np.random.default_rng(n + 11)
+    rows = 5
+    pred, obs = rng.standard_normal((2, rows, n))
+    mask = rng.random((rows, n))
+    want, gwant = of.band_limited_mse(pred, obs, mask, 0.002, 60.0, 0.3, True)
+    ywant = of.lowpass(pred, 0.002, 60.0, 0.3)
+    for dtype, tol in ((torch.float32, 1e-5), (torch.float64, 1e-11)):
+        if dtype == torch.float64 and n > 4096:
+            continue
+        p = torch.tensor(pred, dtype=dtype, device=DEV, requires_grad=True)
+        o = torch.tensor(obs, dtype=dtype, device=DEV)
+        m = torch.tensor(mask, dtype=dtype, device=DEV)
+        loss = BandLimitedMSE(60.0, 0.002)(_Ctx(p, _Obs(o, m)))
+        (g,) = torch.autograd.grad(loss, p)
+        y = lowpass(p.detach(), 0.002, 60.0)
+        el = abs(float(loss) - want) / abs(want)
+        eg = rel(g.double().cpu().numpy(), gwant)
+        ey = rel(y.double().cpu().numpy(), ywant)
+        print(f"n={n} {dtype}: loss rel {el:.1e}, gradient rel-L2 {eg:.2e}, lowpass {ey:.2e}",
+              flush=True)
+        assert el < tol and eg < tol and ey < tol
+
+
 def test_misfit_sums_are_deterministic():
     rng = np.random.default_rng(6)
     p = torch.tensor(rng.standard_normal((300, 2001)), device=DEV, dtype=torch.float32)
