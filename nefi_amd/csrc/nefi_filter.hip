@@ -153,8 +153,8 @@ struct FftPlan {
   static constexpr int LDS_BYTES = ((1 << LOG2L) + ((1 << LOG2L) >> R_LAST)) * 2 * (int)sizeof(T);
   static constexpr int WAVES = (160 * 1024 / LDS_BYTES) >= 2 ? 2 : 1;
   // the top pass's weights held from the start of a pair (32 VGPRs at most)
-  static constexpr int MASK_REGS = 2 * (((1 << LOG2L) >> R_TOP) + kFilterThreads - 1) /
-                                   kFilterThreads * (1 << (R_TOP - 1)) * (int)sizeof(T) / 4;
+  static constexpr int TOP_KG = (((1 << LOG2L) >> R_TOP) + kFilterThreads - 1) / kFilterThreads;
+  static constexpr int MASK_REGS = 2 * TOP_KG * (1 << (R_TOP - 1)) * (int)sizeof(T) / 4;
   static constexpr bool MASK_AHEAD = MASK_REGS <= 32;
 };
 
