@@ -166,6 +166,14 @@ __device__ __forceinline__ int slot(int e) {
   return e + (e >> FftPlan<T, LOG2L>::R_LAST);
 }
 
+// Slot distance of elements d apart when d is a multiple of the pad interval 2^R_LAST: then
+// slot(e + d) = slot(e) + d + d / 2^R_LAST for every e, so a group's slots are its first slot
+// plus compile-time offsets (immediates of the LDS instructions, no per-element arithmetic).
+template <typename T, int LOG2L>
+__device__ __forceinline__ constexpr int slot_step(int d) {
+  return d + (d >> FftPlan<T, LOG2L>::R_LAST);
+}
+
 // exp(-2 pi i j / 32) for j < 16 (cos, -sin): the constant factors of a group's twiddles
 constexpr double kRoot32[16][2] = {
     {1.0, 0.0},
@@ -195,6 +203,41 @@ __device__ __forceinline__ V unit_root(int k, int m) {
   return r;
 }
 
+// Raw buffer loads over one trace row: a resource of `bytes` bytes (stride 0) returns zero for a
+// load past its end, which is the transform's zero padding, and a resource of 0 bytes stands for
+// the missing second row of an odd row count. Word 3 selects 32-bit data (a zero data format
+// would make every load read zero).
+constexpr int kRsrcWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, bytes, kRsrcWord3);
+}
+
+template <typename T>
+__device__ __forceinline__ T row_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Twiddle loads from the table through a buffer resource (SGPRs): a load is the lane's byte
+// offset and nothing else, no 64-bit address arithmetic per twiddle.
+template <typename T>
+__device__ __forceinline__ typename Cplx<T>::type tw_load(__amdgpu_buffer_rsrc_t r, int voff) {
+  using V = typename Cplx<T>::type;
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+  else
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
+
+template <typename T, int LOG2L>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tw_rsrc(const typename Cplx<T>::type* tw) {
+  return row_rsrc(tw, (int)sizeof(typename Cplx<T>::type) << (LOG2L - 1));  // L/2 entries
+}
+
 // The R consecutive radix-2 stages of one group, in registers. A group is the 2^R elements
 // base + q s (q < 2^R) of a block of 2^(ls + R) elements, s = 2^ls: the stages of spans
 // s 2^(R-1) down to s (forward) or s up to s 2^(R-1) (inverse) combine only them. Forward
@@ -204,17 +247,17 @@ __device__ __forceinline__ V unit_root(int k, int m) {
 // table's exp(-2 pi i off / (2h)) times the constant exp(-pi i m s / h): one twiddle load per
 // stage and group, none when the group starts its block (OFF false: off = 0).
 template <typename T, int LOG2L, int R, bool INV, bool OFF>
-__device__ __forceinline__ void group_stages(typename Cplx<T>::type* e,
-                                             const typename Cplx<T>::type* tw, int ls,
-                                             int off) {
+__device__ __forceinline__ void group_stages(typename Cplx<T>::type* e, __amdgpu_buffer_rsrc_t twr,
+                                             int ls, int off) {
   using V = typename Cplx<T>::type;
   constexpr int Q = 1 << R;
+  constexpr int VS = sizeof(T) == 4 ? 3 : 4;  // log2 of a table entry's bytes
   static_assert(R <= 5, "the constant roots cover spans up to 16 in a group");
   V tb[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     if (OFF) {
-      tb[k] = tw[off << (LOG2L - ls - k - 1)];
+      tb[k] = tw_load<T>(twr, off << (LOG2L - ls - k - 1 + VS));
     } else {
       tb[k].x = T(1);
       tb[k].y = T(0);
@@ -253,6 +296,15 @@ __device__ __forceinline__ void group_stages(typename Cplx<T>::type* e,
   }
 }
 
+// threadIdx.x as a value the compiler cannot see through: the group and twiddle index
+// arithmetic of a pass is then computed where it is used, not hoisted out of the loop over trace
+// pairs (where its many long-lived values would not fit in registers)
+__device__ __forceinline__ int thread_id() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // One pass of R stages whose top span is 2^lh_top, LDS to LDS, CH groups in flight per thread
 template <typename T, int LOG2L, int R, bool INV>
 __device__ __forceinline__ void pass_lds(typename Cplx<T>::type* z,
@@ -262,28 +314,31 @@ __device__ __forceinline__ void pass_lds(typename Cplx<T>::type* z,
   constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;  // groups per thread
   constexpr int CH0 = sizeof(T) == 4 ? (R >= 4 ? 2 : 4) : (R >= 3 ? 1 : 2);
   constexpr int CH = KG < CH0 ? KG : CH0;  // groups loaded before the first store
-  const int ls = lh_top - (R - 1);
+  const int ls = lh_top - (R - 1);  // >= R_LAST: the passes above the merged one
   const int s = 1 << ls;
+  const auto twr = tw_rsrc<T, LOG2L>(tw);
 #pragma unroll
   for (int c0 = 0; c0 < KG; c0 += CH) {
     V e[CH][Q];
     int base[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const int g = threadIdx.x + kFilterThreads * (c0 + c);
+      const int g = thread_id() + kFilterThreads * (c0 + c);
       base[c] = -1;
       if (G % kFilterThreads == 0 || g < G) {
         base[c] = ((g >> ls) << (ls + R)) + (g & (s - 1));
+        const int zb = slot<T, LOG2L>(base[c]);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) e[c][q] = z[slot<T, LOG2L>(base[c] + q * s)];
+        for (int q = 0; q < Q; ++q) e[c][q] = z[zb + q * slot_step<T, LOG2L>(s)];
       }
     }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       if (base[c] < 0) continue;
-      group_stages<T, LOG2L, R, INV, true>(e[c], tw, ls, base[c] & (s - 1));
+      group_stages<T, LOG2L, R, INV, true>(e[c], twr, ls, base[c] & (s - 1));
+      const int zb = slot<T, LOG2L>(base[c]);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(base[c] + q * s)] = e[c][q];
+      for (int q = 0; q < Q; ++q) z[zb + q * slot_step<T, LOG2L>(s)] = e[c][q];
     }
   }
 }
@@ -297,28 +352,9 @@ struct TopPass {
   static constexpr int Q = 1 << R, QH = Q / 2, G = (1 << LOG2L) / Q, ls = LOG2L - R;
   static constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;  // groups per thread
   __device__ static bool valid(int c) {
-    return G % kFilterThreads == 0 || (int)threadIdx.x + kFilterThreads * c < G;
+    return G % kFilterThreads == 0 || (int)thread_id() + kFilterThreads * c < G;
   }
 };
-
-// Raw buffer loads over one trace row: a resource of `bytes` bytes (stride 0) returns zero for a
-// load past its end, which is the transform's zero padding, and a resource of 0 bytes stands for
-// the missing second row of an odd row count. Word 3 selects 32-bit data (a zero data format
-// would make every load read zero).
-constexpr int kRsrcWord3 = 0x00020000;
-
-template <typename T>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T* base, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, bytes, kRsrcWord3);
-}
-
-template <typename T>
-__device__ __forceinline__ T row_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  if constexpr (sizeof(T) == 4)
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-  else
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
 
 // The first forward pass: its groups' elements come straight from the traces (the residual
 // x - x_sub; rows r0, r0 + 1 as the real and imaginary parts, zero past the trace length). Lane
@@ -338,7 +374,7 @@ __device__ __forceinline__ void pass_top_load_t(typename Cplx<T>::type* z,
 #pragma unroll
   for (int c = 0; c < P::KG; ++c) {
     if (!P::valid(c)) continue;
-    const int g = threadIdx.x + kFilterThreads * c;
+    const int g = thread_id() + kFilterThreads * c;
     const int voff = g * (int)sizeof(T);
     T a[P::QH], b[P::QH], sa[P::QH], sb[P::QH];
 #pragma unroll
@@ -362,9 +398,10 @@ __device__ __forceinline__ void pass_top_load_t(typename Cplx<T>::type* z,
       e[q + P::QH].x = T(0);
       e[q + P::QH].y = T(0);
     }
-    group_stages<T, LOG2L, R, false, true>(e, tw, P::ls, g);
+    group_stages<T, LOG2L, R, false, true>(e, tw_rsrc<T, LOG2L>(tw), P::ls, g);
+    const int zb = slot<T, LOG2L>(g);
 #pragma unroll
-    for (int q = 0; q < P::Q; ++q) z[slot<T, LOG2L>(g + (q << P::ls))] = e[q];
+    for (int q = 0; q < P::Q; ++q) z[zb + q * slot_step<T, LOG2L>(1 << P::ls)] = e[q];
   }
 }
 
@@ -392,7 +429,7 @@ struct TopMask {
     const auto m1 = row_rsrc(io.mask ? io.mask + o1 : io.x, io.mask && two ? bytes : 0);
 #pragma unroll
     for (int c = 0; c < P::KG; ++c) {
-      const int g = threadIdx.x + kFilterThreads * c;
+      const int g = thread_id() + kFilterThreads * c;
 #pragma unroll
       for (int q = 0; q < P::QH; ++q) {
         const int t = g + (q << P::ls);
@@ -423,11 +460,12 @@ __device__ __forceinline__ void pass_top_store(const typename Cplx<T>::type* z,
 #pragma unroll
   for (int c = 0; c < P::KG; ++c) {
     if (!P::valid(c)) continue;
-    const int g = threadIdx.x + kFilterThreads * c;
+    const int g = thread_id() + kFilterThreads * c;
     V e[P::Q];
+    const int zb = slot<T, LOG2L>(g);
 #pragma unroll
-    for (int q = 0; q < P::Q; ++q) e[q] = z[slot<T, LOG2L>(g + (q << P::ls))];
-    group_stages<T, LOG2L, R, true, true>(e, tw, P::ls, g);
+    for (int q = 0; q < P::Q; ++q) e[q] = z[zb + q * slot_step<T, LOG2L>(1 << P::ls)];
+    group_stages<T, LOG2L, R, true, true>(e, tw_rsrc<T, LOG2L>(tw), P::ls, g);
 #pragma unroll
     for (int q = 0; q < P::QH; ++q) {
       const int t = g + (q << P::ls);
@@ -455,22 +493,23 @@ __device__ __forceinline__ void pass_merged(typename Cplx<T>::type* z,
   constexpr int KG = (G + kFilterThreads - 1) / kFilterThreads;
 #pragma unroll 1
   for (int c = 0; c < KG; ++c) {
-    const int g = threadIdx.x + kFilterThreads * c;
+    const int g = thread_id() + kFilterThreads * c;
     if (G % kFilterThreads != 0 && g >= G) break;
     const int base = g << R;
     V e[Q];
+    const int zb = slot<T, LOG2L>(base);  // base is a multiple of 2^R: its slots are zb + q
 #pragma unroll
-    for (int q = 0; q < Q; ++q) e[q] = z[slot<T, LOG2L>(base + q)];
-    group_stages<T, LOG2L, R, false, false>(e, tw, 0, 0);
+    for (int q = 0; q < Q; ++q) e[q] = z[zb + q];
+    group_stages<T, LOG2L, R, false, false>(e, tw_rsrc<T, LOG2L>(tw), 0, 0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const T h = spec[base + q];
       e[q].x = e[q].x * h;
       e[q].y = e[q].y * h;
     }
-    group_stages<T, LOG2L, R, true, false>(e, tw, 0, 0);
+    group_stages<T, LOG2L, R, true, false>(e, tw_rsrc<T, LOG2L>(tw), 0, 0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) z[slot<T, LOG2L>(base + q)] = e[q];
+    for (int q = 0; q < Q; ++q) z[zb + q] = e[q];
   }
 }
 
