@@ -33,7 +33,15 @@ namespace {
 __device__ __forceinline__ float4 ld4(const float* p) {
   return *reinterpret_cast<const float4*>(p);
 }
+// Wavefield stores are nontemporal: no kernel reads back what it writes, and the next reader
+// is another launch, so the lines need not stay in the caches (measured: 5-6 % faster
+// gradients at 1064^2 x 64 shots, sponge; 2 % with PML).
 __device__ __forceinline__ void st4(float* p, const float4& v) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+}
+__device__ __forceinline__ void st4_lds(float* p, const float4& v) {
   *reinterpret_cast<float4*>(p) = v;
 }
 __device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
@@ -1130,10 +1138,10 @@ __device__ __forceinline__ void zero_margin_rows(float* lpa, float* lpb) {
     lpb[(S::lrows - R) * kLW + i] = 0.f;
   }
   for (int r = R + threadIdx.x; r < S::lrows - R; r += S::threads) {
-    st4(lpa + r * kLW, f4(0.f));
-    st4(lpa + r * kLW + kLW - 4, f4(0.f));
-    st4(lpb + r * kLW, f4(0.f));
-    st4(lpb + r * kLW + kLW - 4, f4(0.f));
+    st4_lds(lpa + r * kLW, f4(0.f));
+    st4_lds(lpa + r * kLW + kLW - 4, f4(0.f));
+    st4_lds(lpb + r * kLW, f4(0.f));
+    st4_lds(lpb + r * kLW + kLW - 4, f4(0.f));
   }
 }
 
