@@ -33,6 +33,15 @@ namespace {
 __device__ __forceinline__ float4 ld4(const float* p) {
   return *reinterpret_cast<const float4*>(p);
 }
+// Nontemporal loads for values read once per launch at a point (the oldest level of the
+// leapfrog): the streaming hint keeps them from displacing the tiles that neighbouring
+// workgroups read again as halos (measured: reverse steps 260-269 -> 234-235 us at 1064^2 x 64
+// shots; in the blocked forward kernel, whose regions overlap, it was 1 % slower and is not used).
+__device__ __forceinline__ float4 ld4nt(const float* p) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
 // Wavefield stores are nontemporal: no kernel reads back what it writes, and the next reader
 // is another launch, so the lines need not stay in the caches (measured: 5-6 % faster
 // gradients at 1064^2 x 64 shots, sponge; 2 % with PML).
@@ -421,7 +430,7 @@ __global__ void __launch_bounds__(256) k2_adjoint(Geo<float> g, AdjIO<float> io,
       const float4 ln = row4(L1, y + R, D1, P2, x, act);
       const float4 mn = row4(MA, y + R, D1, P2, x, act);
       const float4 pn = Pn ? row4(Pn, y + R, D1, P2, x, act) : f4(0.f);
-      const float4 l2c = act ? ld4(io.lam2 + base + ro) : f4(0.f);
+      const float4 l2c = act ? ld4nt(io.lam2 + base + ro) : f4(0.f);
       const float4 lc = act ? ld4(L1 + ro) : f4(0.f);  // read R rows ago: a cache hit
       const float2 el = edge2(L1, y, D2, P2, x, lane);
       const float2 em = edge2(MA, y, D2, P2, x, lane);
@@ -566,7 +575,7 @@ __global__ void __launch_bounds__(256) k3_step(Geo<float> g, StepIO<float> io, i
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
-      pp[k] = (act && y < D1) ? ld4(io.p_prev + base + (y * P2 + x)) : f4(0.f);
+      pp[k] = (act && y < D1) ? ld4nt(io.p_prev + base + (y * P2 + x)) : f4(0.f);
     }
     __syncthreads();  // the previous shot's reads of lp are done
     tile_store<TY, R>(lp, v);
@@ -732,7 +741,7 @@ __device__ __forceinline__ void k3_adj_tile(const Geo<float>& g, const AdjIO<flo
       const int y = y0 + r0 + k;
       const bool ok = Q || (act && y < D1);
       lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);  // also in the tile: a cache hit
-      l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+      l2c[k] = ok ? ld4nt(io.lam2 + base + (y * P2 + x)) : f4(0.f);
       lpn[k] = (ok && imaging && kept_lap) ? ld4(io.lap_n + base + (y * P2 + x)) : f4(0.f);
     }
     lds_barrier();  // the previous shot's reads of lq / lp are done
@@ -1299,7 +1308,7 @@ __device__ __forceinline__ void k5_tile(const Geo<float>& g, const AdjBlockIO& i
 #pragma unroll
     for (int k = 0; k < S::mr; ++k) {
       const int y = ry0 + c0 + k;
-      dst[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+      dst[k] = (Q || (inx && y >= 0 && y < D1)) ? ld4nt(io.lam2 + base + (y * P2 + x)) : f4(0.f);
     }
   };
   // kept Laplacians of every level for the wavefront's output rows, loaded with the shot's
@@ -1657,7 +1666,7 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
 #pragma unroll
     for (int k = 0; k < S::my_rows; ++k) {
       const int y = y0 + r0 + k;
-      pp[k] = (act && y < D1) ? ld4(io.p_prev + base + (y * P2 + x)) : f4(0.f);
+      pp[k] = (act && y < D1) ? ld4nt(io.p_prev + base + (y * P2 + x)) : f4(0.f);
     }
     // The face fields the wave's rows read, issued with the tile so that a shot costs one
     // memory round trip: y-face rows y0+r0 .. y0+r0+my_rows (cell row k reads face rows k and
@@ -1861,7 +1870,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
       const int y = y0 + r0 + k;
       const bool ok = act && y < D1;
       lc[k] = ok ? ld4(L1 + (y * P2 + x)) : f4(0.f);
-      l2c[k] = ok ? ld4(io.lam2 + base + (y * P2 + x)) : f4(0.f);
+      l2c[k] = ok ? ld4nt(io.lam2 + base + (y * P2 + x)) : f4(0.f);
       lpn[k] = (ok && kept_lap && imaging) ? ld4(io.lap_n + base + (y * P2 + x)) : f4(0.f);
       const bool faces = !Q && ok && !(col_quiet && y >= g.quiet[1][0] && y < g.quiet[1][1]);
       i0[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)y * D2 + x)) : f4(0.f);
@@ -2279,7 +2288,7 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
       const float4 lap = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
       const int64_t ro = (int64_t)z * plane + (y * P2 + x);
       const float4 pc = lrow(pz, rr, lane);
-      const float4 pp = ld4(io.p_prev + base + ro);
+      const float4 pp = ld4nt(io.p_prev + base + ro);
       const float4 mm = ld4(io.m + ro);
       const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
       const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
@@ -2357,7 +2366,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
       const int rr = r0 + k + R;
       const float4 lapq = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
       const int64_t ro = (int64_t)z * plane + (y * P2 + x);
-      const float4 lc = lq[0][k], l2c = ld4(io.lam2 + base + ro);
+      const float4 lc = lq[0][k], l2c = ld4nt(io.lam2 + base + ro);
       const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
       const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
       float4 rb;
