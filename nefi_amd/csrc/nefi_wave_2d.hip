@@ -1689,7 +1689,7 @@ __device__ __forceinline__ void k3_step_pml_tile(const Geo<float>& g, const Step
       for (int k = 0; k < S::my_rows; ++k) {
         const float* ox_row = io.psi_old[2] + fx + (int64_t)(y0 + r0 + k) * F2 + x;
         const bool faces = row_faces(k);
-        oxv[k] = faces ? ld4(ox_row) : f4(0.f);
+        oxv[k] = faces ? ld4nt(ox_row) : f4(0.f);  // read once: streaming
         oxe[k] = (faces && own_edge && x + 4 <= D2) ? ox_row[4] : 0.f;
       }
     }
@@ -1877,7 +1877,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
       i1[k] = faces ? ld4(io.psib_in[1] + fy + ((int64_t)(y + 1) * D2 + x)) : f4(0.f);
       // x faces x..x+3, and x+4 where the next lane does not hold it
       const float* in_row = io.psib_in[2] + fx + (int64_t)y * F2 + x;
-      ix[k] = faces ? ld4(in_row) : f4(0.f);
+      ix[k] = faces ? ld4nt(in_row) : f4(0.f);  // read once: streaming
       ixe[k] = (faces && own_edge && x + 4 <= D2) ? in_row[4] : 0.f;
     }
     lds_barrier();  // the previous shot's reads of lq / lp are done
@@ -1992,7 +1992,7 @@ __device__ __forceinline__ void k3_adjoint_pml_tile(const Geo<float>& g, const A
               // face fields' values in the box, to the shuffle and add nothing themselves)
               const float* hr = io.psi_half[2] + fx + (int64_t)y * F2 + x;
               const float* wr = io.psi_low[2] + fx + (int64_t)y * F2 + x;
-              const float4 h4 = lqr ? f4(0.f) : ld4(hr), w4 = lqr ? f4(0.f) : ld4(wr);
+              const float4 h4 = lqr ? f4(0.f) : ld4nt(hr), w4 = lqr ? f4(0.f) : ld4nt(wr);
               const bool e = !lqr && own_edge && x + 4 <= D2;
               float hw[5];
               faces5(make_float4(h4.x + w4.x, h4.y + w4.y, h4.z + w4.z, h4.w + w4.w),
