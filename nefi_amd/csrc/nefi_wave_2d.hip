@@ -53,6 +53,14 @@ __device__ __forceinline__ void st4(float* p, const float4& v) {
 __device__ __forceinline__ void st4_lds(float* p, const float4& v) {
   *reinterpret_cast<float4*>(p) = v;
 }
+// Plain wavefield store, for the 3-D kernels: a 3-D photoacoustic field (212^3, one shot, 38 MB)
+// stays in the 256 MiB last-level cache from one launch to the next, and the nontemporal hints
+// cost that workload 7 % (88.9 -> 95.7 ms per forward + adjoint). A size-dependent choice made
+// at run time inside the kernels broke the 2-D kernels' load scheduling, so the choice is per
+// kernel family: 2-D plans at the benchmark scale (64 shots, 290 MB per field) stream.
+__device__ __forceinline__ void st4p(float* p, const float4& v) {
+  *reinterpret_cast<float4*>(p) = v;
+}
 __device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
 
 // Read-only table value at a wave-uniform index (profiles, source taps): through the constant
@@ -2288,7 +2296,7 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
       const float4 lap = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
       const int64_t ro = (int64_t)z * plane + (y * P2 + x);
       const float4 pc = lrow(pz, rr, lane);
-      const float4 pp = ld4nt(io.p_prev + base + ro);
+      const float4 pp = ld4(io.p_prev + base + ro);
       const float4 mm = ld4(io.m + ro);
       const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
       const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
@@ -2308,11 +2316,11 @@ __global__ void __launch_bounds__(256) k3d_step(Geo<float> g, StepIO<float> io, 
         }
       }
       if (ABS != NW_NONE) o = mul4(dm.a, o);
-      st4(io.p_next + base + ro, o);
-      if (io.lap_out) st4(io.lap_out + base + ro, lap);
+      st4p(io.p_next + base + ro, o);
+      if (io.lap_out) st4p(io.lap_out + base + ro, lap);
       if (io.ck_prev) {
-        st4(io.ck_prev + base + ro, pp);
-        st4(io.ck_cur + base + ro, pc);
+        st4p(io.ck_prev + base + ro, pp);
+        st4p(io.ck_cur + base + ro, pc);
       }
     }
   }, [](int) {});
@@ -2366,7 +2374,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
       const int rr = r0 + k + R;
       const float4 lapq = lap3<R>(g, pz, pm1, pp1, pm2, pp2, rr, lane);
       const int64_t ro = (int64_t)z * plane + (y * P2 + x);
-      const float4 lc = lq[0][k], l2c = ld4nt(io.lam2 + base + ro);
+      const float4 lc = lq[0][k], l2c = ld4(io.lam2 + base + ro);
       const float hy = ABS != NW_NONE ? g.damp[1][y] : 0.f;
       const Damp4<ABS> dm = damp3_4<ABS>(hz, hy, hx);
       float4 rb;
@@ -2375,7 +2383,7 @@ __global__ void __launch_bounds__(256) k3d_adjoint(Geo<float> g, AdjIO<float> io
           adj1<ABS>(lc.y, l2c.y, lapq.y, dm.a.y, dm.b.y, dm.tap.y, tpn, rb.y),
           adj1<ABS>(lc.z, l2c.z, lapq.z, dm.a.z, dm.b.z, dm.tap.z, tpn, rb.z),
           adj1<ABS>(lc.w, l2c.w, lapq.w, dm.a.w, dm.b.w, dm.tap.w, tpn, rb.w));
-      st4(io.lam0 + base + ro, o);
+      st4p(io.lam0 + base + ro, o);
       if (imaging) {
         const float4 lapp = ld4(io.lap_n + base + ro);
         double2* gp = reinterpret_cast<double2*>(io.grad + base + ro);
